@@ -1,0 +1,175 @@
+"""ctypes binding of libcaustica_hip.so (declarations in include/caustica_hip.h).
+
+The library is the only compute path of this package: loading fails loudly when it
+has not been built, and there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libcaustica_hip.so"
+
+CST_OK = 0
+CST_ERR_ARG = 1
+CST_ERR_HIP = 2
+CST_ERR_UNSUPPORTED = 3
+CST_ERR_STATE = 4
+MAX_BATCH = 16
+MAX_HARMONICS = 3
+
+#: Every symbol include/caustica_hip.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "cst_version",
+    "cst_last_error",
+    "cst_device_count",
+    "cst_supported_length",
+    "cst_session_create",
+    "cst_session_destroy",
+    "cst_session_stream",
+    "cst_member_add",
+    "cst_member_remove",
+    "cst_member_schedule",
+    "cst_member_step",
+    "cst_advance",
+    "cst_sync",
+    "cst_member_peaks",
+    "cst_member_result",
+    "cst_member_get_state",
+    "cst_member_set_state",
+    "cst_session_set_profiling",
+    "cst_session_kernel_time",
+)
+
+
+class NativeError(RuntimeError):
+    """A call into libcaustica_hip returned a nonzero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[status {code}] {msg}")
+        self.code = code
+
+
+class UnsupportedShape(NativeError):
+    pass
+
+
+class MemberDesc(C.Structure):
+    _fields_ = [
+        ("dx", C.c_double),
+        ("pml_vox", C.c_int),
+        ("pml_edge", C.c_double),
+        ("dt", C.c_double),
+        ("c_ref", C.c_double),
+        ("spp", C.c_int),
+        ("f0", C.c_double),
+        ("amplitude", C.c_double),
+        ("ramp_periods", C.c_double),
+        ("dt_over_rho", C.c_void_p),
+        ("rhoc2_dt", C.c_void_p),
+        ("absorb", C.c_void_p),
+        ("beta2_dt", C.c_void_p),
+        ("ktab", C.c_void_p),
+        ("sponge", C.c_void_p),
+        ("n_src", C.c_int),
+        ("src_idx", C.c_void_p),
+        ("src_phase", C.c_void_p),
+        ("src_scale", C.c_void_p),
+        ("conv", C.c_int * 6),
+        ("rec", C.c_int * 6),
+        ("n_harmonics", C.c_int),
+        ("harmonics", C.c_int * MAX_HARMONICS),
+        ("max_periods", C.c_int),
+    ]
+
+
+class ThermalDesc(C.Structure):
+    _fields_ = [
+        ("dx", C.c_double),
+        ("k", C.c_void_p),
+        ("perf", C.c_void_p),
+        ("coeff", C.c_void_p),
+        ("q", C.c_void_p),
+        ("theta0", C.c_void_p),
+        ("dose0", C.c_void_p),
+        ("boundary", C.c_int),
+        ("theta_wall", C.c_float),
+        ("t_ref", C.c_float),
+        ("half_dt_min", C.c_double),
+        ("dose", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def _bind(lib):
+    i, v, d, p = C.c_int, C.c_void_p, C.c_double, C.c_void_p
+    i64 = C.c_int64
+    sig = {
+        "cst_version": ([], i),
+        "cst_last_error": ([C.c_char_p, C.c_size_t], i),
+        "cst_device_count": ([C.POINTER(i)], i),
+        "cst_supported_length": ([i, i], i),
+        "cst_session_create": ([i, i, i, i, i, C.POINTER(v)], i),
+        "cst_session_destroy": ([v], i),
+        "cst_session_stream": ([v, C.POINTER(v)], i),
+        "cst_member_add": ([v, C.POINTER(MemberDesc), C.POINTER(i)], i),
+        "cst_member_remove": ([v, i], i),
+        "cst_member_schedule": ([v, i, i64], i),
+        "cst_member_step": ([v, i, C.POINTER(i64)], i),
+        "cst_advance": ([v, i, C.POINTER(i), i], i),
+        "cst_sync": ([v], i),
+        "cst_member_peaks": ([v, i, i, i, p], i),
+        "cst_member_result": ([v, i, i, d, p, p], i),
+        "cst_member_get_state": ([v, i, p, p, p, p, C.POINTER(i64)], i),
+        "cst_member_set_state": ([v, i, i64, p, p, p, p], i),
+        "cst_session_set_profiling": ([v, i], i),
+        "cst_session_kernel_time": ([v, i, C.POINTER(d), C.POINTER(i64)], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+def lib():
+    """The loaded library; raises if it was never built (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m caustica_amd.build` "
+                f"(hipcc, gfx950). caustica_amd has no CPU fallback."
+            )
+        _lib = _bind(C.CDLL(os.fspath(LIB_PATH)))
+    return _lib
+
+
+def last_error() -> str:
+    buf = C.create_string_buffer(4096)
+    lib().cst_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(code: int) -> None:
+    if code != CST_OK:
+        msg = last_error()
+        if code == CST_ERR_UNSUPPORTED:
+            raise UnsupportedShape(code, msg)
+        raise NativeError(code, msg)
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(lib().cst_device_count(C.byref(n)))
+    return int(n.value)

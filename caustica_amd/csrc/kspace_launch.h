@@ -1,0 +1,25 @@
+// Launchers for the k-space pass kernels over the instantiated axis lengths.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kspace_types.h"
+
+namespace cst {
+
+// Lengths with instantiated kernels: x/y passes use L = nx or ny, z passes use M = nz / 2.
+#define CST_LENGTHS(X)                                                                       \
+  X(1) X(2) X(4) X(6) X(8) X(10) X(12) X(15) X(16) X(20) X(24) X(30) X(32) X(36) X(40) X(48) \
+  X(60) X(64) X(72) X(80) X(96) X(100) X(120) X(128) X(144) X(150) X(160) X(192) X(200)      \
+  X(240) X(256) X(288) X(300) X(320) X(384) X(400) X(512)
+
+bool length_supported(int L);
+
+// which: 0 -> sp (one spectrum), 1 -> sg[0..2] (three spectra)
+bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, hipStream_t st);
+bool launch_xgrad(int L, const StepArgs& a, const float2* tw, hipStream_t st);
+bool launch_xdiv(int L, const StepArgs& a, const float2* tw, hipStream_t st);
+bool launch_zu(int M, const StepArgs& a, const float2* tw, const float2* rt, hipStream_t st);
+bool launch_zp(int M, const StepArgs& a, const float2* tw, const float2* rt, int update,
+               hipStream_t st);
+
+}  // namespace cst

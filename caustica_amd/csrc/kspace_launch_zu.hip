@@ -1,0 +1,24 @@
+// Template instantiation + launch of the k-space pass kernels: z velocity pass.
+#include "kspace_kernels.h"
+#include "kspace_launch.h"
+
+namespace cst {
+
+bool launch_zu(int M, const StepArgs& a, const float2* tw, const float2* rt, hipStream_t st) {
+  const int nl = a.s.nx * a.s.ny;
+  switch (M) {
+#define CST_ZU(N)                                                                      \
+  case N: {                                                                            \
+    using Z = ZPlan<N>;                                                                \
+    dim3 g((nl + Z::LPB - 1) / Z::LPB, 1, a.nmem);                                     \
+    hipLaunchKernelGGL((k_zu<N>), g, dim3(Z::THREADS), 0, st, a, tw, rt);              \
+    return true;                                                                       \
+  }
+    CST_LENGTHS(CST_ZU)
+#undef CST_ZU
+    default:
+      return false;
+  }
+}
+
+}  // namespace cst

@@ -1,0 +1,67 @@
+// Device-side descriptors shared by the k-space pass kernels and the host session.
+//
+// Layout in HBM for one solve ("member") on a padded grid nx x ny x nz (z contiguous):
+//   real volumes   [nx][ny][nz]      float     p, u0, u1, u2, dt/rho, rho c^2 dt, absorb, 2 beta dt
+//   half spectra   [nx][ny][nz/2+1]  float2    sp (p-hat, then the divergence spectrum),
+//                                              sg0..sg2 (gradient spectra, then u-hat)
+//   record box     [rx][ry][rz]      float2    Kahan sum + compensation per harmonic; float p_max
+// Several members of one shape share every launch (blockIdx.z indexes the member list).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cst {
+
+constexpr int kMaxBatch = 16;     // members per launch
+constexpr int kMaxHarmonics = 3;  // harmonics recorded per member
+
+struct MemberDev {
+  float* p;
+  float* u[3];
+  float2* sp;
+  float2* sg[3];
+  const float* dto_rho;
+  const float* rhoc2dt;
+  const float* absorb;
+  const float* beta2dt;  // null: linear update
+  // k tables: kx_full[nx] kx_d[nx] ky_full[ny] ky_d[ny] kz_full[nzh] kz_d[nzh]
+  // (_d = derivative wavenumber with the even-axis Nyquist bin zeroed)
+  const float* ktab;
+  const float* sponge;   // sz[nz] sx[nx] sy[ny] (sz first: float2-aligned)
+  const int* src_ptr;    // [nx*ny + 1] CSR over (x, y) lines
+  const int* src_z;      // z index of each source voxel, grouped by line
+  const float2* src_ab;  // (scale cos(phase), scale sin(phase)) per source voxel
+  float2* acc[kMaxHarmonics];
+  float2* comp[kMaxHarmonics];
+  float* pmax;
+  unsigned* peaks;       // per-period max |p| over the convergence box (float bits)
+  int conv[6];           // x0, x1, y0, y1, z0, z1 (half-open)
+  int rec[6];
+  int nh;
+  float half_cdt;        // c_ref * dt / 2
+  float inv_n;           // 1 / (nx ny nz): the inverse-transform normalization
+  int pad_;
+};
+
+struct StepMember {
+  float amp;      // float32(amplitude * ramp envelope at t = n dt)
+  float sin_t;    // sin(2 pi (n mod spp) / spp)
+  float cos_t;
+  int record;     // accumulate phasors / p_max this step
+  float2 eh[kMaxHarmonics];  // exp(+i h 2 pi (n mod spp) / spp)
+  int peak_idx;   // period slot for the convergence peak, -1: none
+  int pad_;
+};
+
+struct Shape {
+  int nx, ny, nz, nzh;
+};
+
+struct StepArgs {
+  const MemberDev* mem;
+  Shape s;
+  int nmem;
+  int ids[kMaxBatch];
+  StepMember sm[kMaxBatch];
+};
+
+}  // namespace cst

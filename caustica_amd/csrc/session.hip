@@ -1,0 +1,518 @@
+// Host session of the CW k-space solver: device memory per member, step scalars,
+// batched launch sequence, results, checkpoint state, kernel timing. C ABI in
+// include/caustica_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/caustica_hip.h"
+#include "common.h"
+#include "kspace_launch.h"
+#include "kspace_types.h"
+
+using namespace cst;
+
+namespace {
+
+struct Member {
+  bool used = false;
+  int64_t n = 0;
+  int64_t record_from = INT64_MAX;
+  double dt = 0, f0 = 0, amplitude = 0, ramp_periods = 3;
+  int spp = 0, nh = 1, harmonics[kMaxHarmonics] = {1, 0, 0};
+  int max_periods = 0;
+  int64_t rec_count = 0;
+  bool finalized[kMaxHarmonics] = {false, false, false};
+  bool primed = true;
+  void* block = nullptr;
+  MemberDev dev{};
+};
+
+__global__ void k_finalize(const float2* __restrict__ acc, float2* __restrict__ comp, int64_t n,
+                           double scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const float2 s = acc[i], c = comp[i];
+    comp[i] = make_float2((float)(((double)s.x - (double)c.x) * scale),
+                          (float)(((double)s.y - (double)c.y) * scale));
+  }
+}
+
+}  // namespace
+
+struct cst_session {
+  int device = 0;
+  Shape s{};
+  int max_members = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Member> members;
+  MemberDev* d_table = nullptr;
+  float2* d_tables = nullptr;  // twiddles: x (nx), y (ny), z (M), real split (M + 1)
+  float2* tw_x = nullptr;
+  float2* tw_y = nullptr;
+  float2* tw_z = nullptr;
+  float2* rt = nullptr;
+  Profiler prof;
+};
+
+namespace {
+
+int fill_twiddles(std::vector<float2>& out, int L) {
+  for (int e = 0; e < L; ++e) {
+    const double ang = 2.0 * M_PI * (double)e / (double)L;
+    out.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+  }
+  return L;
+}
+
+int check_slot(cst_session* s, int slot) {
+  if (!s) return fail(CST_ERR_ARG, "null session");
+  if (slot < 0 || slot >= s->max_members || !s->members[slot].used)
+    return fail(CST_ERR_STATE, "member slot " + std::to_string(slot) + " is empty");
+  return CST_OK;
+}
+
+void step_scalars(const Member& m, StepMember& sm) {
+  const int64_t n = m.n;
+  const double period = 1.0 / m.f0;
+  const double t = (double)n * m.dt;
+  const double x = std::min(t / (m.ramp_periods * period), 1.0);
+  const double env = 0.5 * (1.0 - std::cos(M_PI * x));
+  sm.amp = (float)(m.amplitude * env);
+  const int64_t ph = n % m.spp;
+  const double th = 2.0 * M_PI * (double)ph / (double)m.spp;
+  sm.sin_t = (float)std::sin(th);
+  sm.cos_t = (float)std::cos(th);
+  sm.record = n >= m.record_from ? 1 : 0;
+  for (int h = 0; h < kMaxHarmonics; ++h) {
+    const int hh = h < m.nh ? m.harmonics[h] : 0;
+    const double a = 2.0 * M_PI * (double)((hh * ph) % m.spp) / (double)m.spp;
+    sm.eh[h] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  const int64_t pidx = n / m.spp;
+  sm.peak_idx = (!sm.record && pidx < m.max_periods) ? (int)pidx : -1;
+}
+
+int launch_step(cst_session* s, StepArgs& a) {
+  const Shape& sh = s->s;
+  const int M = sh.nz / 2;
+  hipStream_t st = s->stream;
+  bool ok = true;
+  ok = ok && s->prof.run(0, st, [&] { return launch_xgrad(sh.nx, a, s->tw_x, st); });
+  if (sh.ny > 1) ok = ok && s->prof.run(1, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 1, st); });
+  ok = ok && s->prof.run(2, st, [&] { return launch_zu(M, a, s->tw_z, s->rt, st); });
+  if (sh.ny > 1) ok = ok && s->prof.run(1, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 1, st); });
+  ok = ok && s->prof.run(3, st, [&] { return launch_xdiv(sh.nx, a, s->tw_x, st); });
+  if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 0, st); });
+  ok = ok && s->prof.run(5, st, [&] { return launch_zp(M, a, s->tw_z, s->rt, 1, st); });
+  if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 0, st); });
+  if (!ok) return fail(CST_ERR_UNSUPPORTED, "no kernel instantiated for this shape");
+  HIPCHK(hipGetLastError());
+  return CST_OK;
+}
+
+// R2C + forward y of the member's current p into sp (after set_state).
+int prime(cst_session* s, int slot) {
+  StepArgs a{};
+  a.mem = s->d_table;
+  a.s = s->s;
+  a.nmem = 1;
+  a.ids[0] = slot;
+  a.sm[0].peak_idx = -1;
+  const int M = s->s.nz / 2;
+  if (!launch_zp(M, a, s->tw_z, s->rt, 0, s->stream))
+    return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
+  if (s->s.ny > 1 && !launch_y(s->s.ny, -1, a, s->tw_y, 0, s->stream))
+    return fail(CST_ERR_UNSUPPORTED, "no kernel for ny");
+  HIPCHK(hipGetLastError());
+  s->members[slot].primed = true;
+  return CST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cst_version(void) { return 100; }
+
+int cst_last_error(char* buf, size_t n) { return copy_last_error(buf, n); }
+
+int cst_device_count(int* n) {
+  if (!n) return fail(CST_ERR_ARG, "null output");
+  HIPCHK(hipGetDeviceCount(n));
+  return CST_OK;
+}
+
+int cst_supported_length(int n, int axis) {
+  if (n < 1) return 0;
+  if (axis == 2) return (n % 2 == 0 && length_supported(n / 2)) ? 1 : 0;
+  return length_supported(n) ? 1 : 0;
+}
+
+int cst_session_create(int nx, int ny, int nz, int max_members, int device, cst_session** out) {
+  if (!out) return fail(CST_ERR_ARG, "null output");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 2 || max_members < 1 || max_members > 4096)
+    return fail(CST_ERR_ARG, "invalid shape or member count");
+  if (!cst_supported_length(nx, 0) || !cst_supported_length(ny, 1) || !cst_supported_length(nz, 2))
+    return fail(CST_ERR_UNSUPPORTED, "padded shape (" + std::to_string(nx) + ", " + std::to_string(ny) +
+                                         ", " + std::to_string(nz) +
+                                         ") has an axis without an instantiated kernel (z must be even)");
+  HIPCHK(hipSetDevice(device));
+  auto* s = new cst_session();
+  s->device = device;
+  s->s = Shape{nx, ny, nz, nz / 2 + 1};
+  s->max_members = max_members;
+  s->members.resize(max_members);
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return fail(CST_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  std::vector<float2> tab;
+  fill_twiddles(tab, nx);
+  fill_twiddles(tab, ny);
+  fill_twiddles(tab, nz / 2);
+  for (int k = 0; k <= nz / 2; ++k) {
+    const double ang = 2.0 * M_PI * (double)k / (double)nz;
+    tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+  }
+  e = hipMalloc(&s->d_tables, tab.size() * sizeof(float2));
+  if (e == hipSuccess) e = hipMalloc(&s->d_table, sizeof(MemberDev) * max_members);
+  if (e == hipSuccess)
+    e = hipMemcpy(s->d_tables, tab.data(), tab.size() * sizeof(float2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(s->d_table, 0, sizeof(MemberDev) * max_members);
+  if (e != hipSuccess) {
+    cst_session_destroy(s);
+    return fail(CST_ERR_HIP, std::string("session tables: ") + hipGetErrorString(e));
+  }
+  s->tw_x = s->d_tables;
+  s->tw_y = s->tw_x + nx;
+  s->tw_z = s->tw_y + ny;
+  s->rt = s->tw_z + nz / 2;
+  *out = s;
+  return CST_OK;
+}
+
+int cst_session_destroy(cst_session* s) {
+  if (!s) return CST_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (auto& m : s->members)
+    if (m.block) (void)hipFree(m.block);
+  if (s->d_table) (void)hipFree(s->d_table);
+  if (s->d_tables) (void)hipFree(s->d_tables);
+  s->prof.release();
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return CST_OK;
+}
+
+int cst_session_stream(cst_session* s, void** stream) {
+  if (!s || !stream) return fail(CST_ERR_ARG, "null argument");
+  *stream = (void*)s->stream;
+  return CST_OK;
+}
+
+int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
+  if (!s || !d || !slot_out) return fail(CST_ERR_ARG, "null argument");
+  const Shape& sh = s->s;
+  if (d->n_harmonics < 1 || d->n_harmonics > kMaxHarmonics)
+    return fail(CST_ERR_ARG, "n_harmonics must be 1..3");
+  if (d->spp < 2 || !(d->dt > 0) || !(d->f0 > 0) || !(d->amplitude > 0) || !(d->ramp_periods > 0))
+    return fail(CST_ERR_ARG, "invalid timing (spp, dt, f0, amplitude, ramp)");
+  if (!d->dt_over_rho || !d->rhoc2_dt || !d->absorb || !d->ktab || !d->sponge)
+    return fail(CST_ERR_ARG, "missing coefficient maps, k tables or sponge profiles");
+  if (d->n_src < 0 || (d->n_src > 0 && (!d->src_idx || !d->src_phase || !d->src_scale)))
+    return fail(CST_ERR_ARG, "invalid source arrays");
+  const int dims[3] = {sh.nx, sh.ny, sh.nz};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int c0 = d->conv[2 * ax], c1 = d->conv[2 * ax + 1];
+    const int r0 = d->rec[2 * ax], r1 = d->rec[2 * ax + 1];
+    if (c0 < 0 || c1 > dims[ax] || c0 >= c1 || r0 < 0 || r1 > dims[ax] || r0 >= r1)
+      return fail(CST_ERR_ARG, "convergence or record box outside the padded grid");
+  }
+  int slot = -1;
+  for (int i = 0; i < s->max_members; ++i)
+    if (!s->members[i].used) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) return fail(CST_ERR_STATE, "session is full");
+
+  // sources: CSR over (x, y) lines, duplicates refused
+  const int nl = sh.nx * sh.ny;
+  std::vector<int> ptr(nl + 1, 0), order(d->n_src);
+  for (int j = 0; j < d->n_src; ++j) {
+    const int32_t* v = d->src_idx + 3 * j;
+    if (v[0] < 0 || v[0] >= sh.nx || v[1] < 0 || v[1] >= sh.ny || v[2] < 0 || v[2] >= sh.nz)
+      return fail(CST_ERR_ARG, "source voxel outside the padded grid");
+    order[j] = j;
+  }
+  auto key = [&](int j) {
+    const int32_t* v = d->src_idx + 3 * j;
+    return ((int64_t)v[0] * sh.ny + v[1]) * sh.nz + v[2];
+  };
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+  for (int j = 1; j < d->n_src; ++j)
+    if (key(order[j]) == key(order[j - 1])) return fail(CST_ERR_ARG, "duplicate source voxel");
+  std::vector<int> src_z(std::max(d->n_src, 1));
+  std::vector<float2> src_ab(std::max(d->n_src, 1));
+  for (int j = 0; j < d->n_src; ++j) {
+    const int o = order[j];
+    const int32_t* v = d->src_idx + 3 * o;
+    ptr[v[0] * sh.ny + v[1] + 1]++;
+    src_z[j] = v[2];
+    const double sc = d->src_scale[o], ph = d->src_phase[o];
+    src_ab[j] = make_float2((float)(sc * std::cos(ph)), (float)(sc * std::sin(ph)));
+  }
+  for (int i = 0; i < nl; ++i) ptr[i + 1] += ptr[i];
+
+  // one device allocation, carved in 256-byte aligned pieces
+  const int64_t P = (int64_t)sh.nx * sh.ny * sh.nz;
+  const int64_t H = (int64_t)sh.nx * sh.ny * sh.nzh;
+  const int64_t R = (int64_t)(d->rec[1] - d->rec[0]) * (d->rec[3] - d->rec[2]) * (d->rec[5] - d->rec[4]);
+  const int nmaps = d->beta2_dt ? 4 : 3;
+  const int64_t nk = 2LL * (sh.nx + sh.ny + sh.nzh);
+  const int64_t nsp = sh.nz + sh.nx + sh.ny;
+  const int maxp = std::max(d->max_periods, 1);
+  Carver c;
+  const size_t o_state = c.take(4 * P * 4);
+  const size_t o_spec = c.take(4 * H * 8);
+  const size_t o_maps = c.take(nmaps * P * 4);
+  const size_t o_k = c.take(nk * 4);
+  const size_t o_sp = c.take(nsp * 4);
+  const size_t o_ptr = c.take((nl + 1) * 4);
+  const size_t o_sz = c.take(src_z.size() * 4);
+  const size_t o_sab = c.take(src_ab.size() * 8);
+  const size_t o_rec = c.take(2 * d->n_harmonics * R * 8);
+  const size_t o_pmax = c.take(R * 4);
+  const size_t o_peaks = c.take(maxp * 4);
+  HIPCHK(hipSetDevice(s->device));
+  Member& m = s->members[slot];
+  char* base = nullptr;
+  HIPCHK(hipMalloc((void**)&base, c.total));
+  m = Member{};
+  m.block = base;
+  m.used = true;
+  MemberDev& md = m.dev;
+  float* st = (float*)(base + o_state);
+  md.p = st;
+  for (int a = 0; a < 3; ++a) md.u[a] = st + (a + 1) * P;
+  float2* spc = (float2*)(base + o_spec);
+  md.sp = spc;
+  for (int a = 0; a < 3; ++a) md.sg[a] = spc + (a + 1) * H;
+  float* maps = (float*)(base + o_maps);
+  md.dto_rho = maps;
+  md.rhoc2dt = maps + P;
+  md.absorb = maps + 2 * P;
+  md.beta2dt = d->beta2_dt ? maps + 3 * P : nullptr;
+  md.ktab = (float*)(base + o_k);
+  md.sponge = (float*)(base + o_sp);
+  md.src_ptr = (int*)(base + o_ptr);
+  md.src_z = (int*)(base + o_sz);
+  md.src_ab = (float2*)(base + o_sab);
+  float2* rec = (float2*)(base + o_rec);
+  for (int h = 0; h < kMaxHarmonics; ++h) {
+    md.acc[h] = h < d->n_harmonics ? rec + (2 * h) * R : nullptr;
+    md.comp[h] = h < d->n_harmonics ? rec + (2 * h + 1) * R : nullptr;
+  }
+  md.pmax = (float*)(base + o_pmax);
+  md.peaks = (unsigned*)(base + o_peaks);
+  for (int i = 0; i < 6; ++i) {
+    md.conv[i] = d->conv[i];
+    md.rec[i] = d->rec[i];
+  }
+  md.nh = d->n_harmonics;
+  md.half_cdt = (float)(0.5 * d->c_ref * d->dt);
+  md.inv_n = (float)(1.0 / (double)P);
+
+  m.dt = d->dt;
+  m.f0 = d->f0;
+  m.amplitude = d->amplitude;
+  m.ramp_periods = d->ramp_periods;
+  m.spp = d->spp;
+  m.nh = d->n_harmonics;
+  for (int h = 0; h < kMaxHarmonics; ++h) m.harmonics[h] = h < d->n_harmonics ? d->harmonics[h] : 0;
+  m.max_periods = maxp;
+
+  hipStream_t stm = s->stream;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stm);
+  };
+  hipError_t e = hipMemsetAsync(base + o_state, 0, 4 * P * 4, stm);
+  if (e == hipSuccess) e = hipMemsetAsync(base + o_spec, 0, 4 * H * 8, stm);
+  if (e == hipSuccess) e = up((void*)md.dto_rho, d->dt_over_rho, P * 4);
+  if (e == hipSuccess) e = up((void*)md.rhoc2dt, d->rhoc2_dt, P * 4);
+  if (e == hipSuccess) e = up((void*)md.absorb, d->absorb, P * 4);
+  if (e == hipSuccess && d->beta2_dt) e = up((void*)md.beta2dt, d->beta2_dt, P * 4);
+  if (e == hipSuccess) e = up((void*)md.ktab, d->ktab, nk * 4);
+  if (e == hipSuccess) e = up((void*)md.sponge, d->sponge, nsp * 4);
+  if (e == hipSuccess) e = up((void*)md.src_ptr, ptr.data(), (nl + 1) * 4);
+  if (e == hipSuccess) e = up((void*)md.src_z, src_z.data(), src_z.size() * 4);
+  if (e == hipSuccess) e = up((void*)md.src_ab, src_ab.data(), src_ab.size() * 8);
+  if (e == hipSuccess) e = hipMemsetAsync(base + o_rec, 0, 2 * d->n_harmonics * R * 8, stm);
+  if (e == hipSuccess) e = hipMemsetAsync(base + o_pmax, 0, R * 4, stm);
+  if (e == hipSuccess) e = hipMemsetAsync(md.peaks, 0, maxp * 4, stm);
+  if (e == hipSuccess) e = up(s->d_table + slot, &md, sizeof(MemberDev));
+  if (e == hipSuccess) e = hipStreamSynchronize(stm);
+  if (e != hipSuccess) {
+    (void)hipFree(base);
+    m = Member{};
+    return fail(CST_ERR_HIP, std::string("member upload: ") + hipGetErrorString(e));
+  }
+  m.rec_count = R;
+  *slot_out = slot;
+  return CST_OK;
+}
+
+int cst_member_remove(cst_session* s, int slot) {
+  if (int r = check_slot(s, slot)) return r;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipFree(s->members[slot].block));
+  s->members[slot] = Member{};
+  return CST_OK;
+}
+
+int cst_member_schedule(cst_session* s, int slot, int64_t record_from) {
+  if (int r = check_slot(s, slot)) return r;
+  s->members[slot].record_from = record_from;
+  return CST_OK;
+}
+
+int cst_member_step(cst_session* s, int slot, int64_t* n) {
+  if (int r = check_slot(s, slot)) return r;
+  if (!n) return fail(CST_ERR_ARG, "null output");
+  *n = s->members[slot].n;
+  return CST_OK;
+}
+
+int cst_advance(cst_session* s, int n_slots, const int* slots, int n_steps) {
+  if (!s || (n_slots > 0 && !slots)) return fail(CST_ERR_ARG, "null argument");
+  if (n_slots < 1 || n_slots > kMaxBatch) return fail(CST_ERR_ARG, "1..16 members per advance");
+  if (n_steps < 0) return fail(CST_ERR_ARG, "negative step count");
+  for (int i = 0; i < n_slots; ++i) {
+    if (int r = check_slot(s, slots[i])) return r;
+    for (int j = 0; j < i; ++j)
+      if (slots[j] == slots[i]) return fail(CST_ERR_ARG, "member listed twice");
+  }
+  HIPCHK(hipSetDevice(s->device));
+  for (int i = 0; i < n_slots; ++i)
+    if (!s->members[slots[i]].primed)
+      if (int r = prime(s, slots[i])) return r;
+  StepArgs a{};
+  a.mem = s->d_table;
+  a.s = s->s;
+  a.nmem = n_slots;
+  for (int i = 0; i < n_slots; ++i) a.ids[i] = slots[i];
+  for (int k = 0; k < n_steps; ++k) {
+    for (int i = 0; i < n_slots; ++i) {
+      Member& m = s->members[slots[i]];
+      step_scalars(m, a.sm[i]);
+      if (a.sm[i].record)
+        for (int h = 0; h < m.nh; ++h) m.finalized[h] = false;
+    }
+    if (int r = launch_step(s, a)) return r;
+    for (int i = 0; i < n_slots; ++i) s->members[slots[i]].n++;
+  }
+  return CST_OK;
+}
+
+int cst_sync(cst_session* s) {
+  if (!s) return fail(CST_ERR_ARG, "null session");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CST_OK;
+}
+
+int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out) {
+  if (int r = check_slot(s, slot)) return r;
+  Member& m = s->members[slot];
+  if (!out || first < 0 || count < 0 || first + count > m.max_periods)
+    return fail(CST_ERR_ARG, "peak range outside the member's period capacity");
+  if (count == 0) return CST_OK;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipMemcpyAsync(out, m.dev.peaks + first, count * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CST_OK;
+}
+
+int cst_member_result(cst_session* s, int slot, int hi, double scale, float* phasor, float* pmax) {
+  if (int r = check_slot(s, slot)) return r;
+  Member& m = s->members[slot];
+  if (hi < 0 || hi >= m.nh) return fail(CST_ERR_ARG, "harmonic index out of range");
+  HIPCHK(hipSetDevice(s->device));
+  const int64_t R = m.rec_count;
+  if (!m.finalized[hi]) {
+    const int threads = 256;
+    const int64_t blocks = (R + threads - 1) / threads;
+    s->prof.run(6, s->stream, [&] {
+      hipLaunchKernelGGL(k_finalize, dim3((unsigned)blocks), dim3(threads), 0, s->stream, m.dev.acc[hi],
+                         m.dev.comp[hi], R, scale);
+      return true;
+    });
+    HIPCHK(hipGetLastError());
+    m.finalized[hi] = true;
+  }
+  if (phasor)
+    HIPCHK(hipMemcpyAsync(phasor, m.dev.comp[hi], R * 8, hipMemcpyDeviceToHost, s->stream));
+  if (pmax) HIPCHK(hipMemcpyAsync(pmax, m.dev.pmax, R * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CST_OK;
+}
+
+int cst_member_get_state(cst_session* s, int slot, float* p, float* u0, float* u1, float* u2,
+                         int64_t* n) {
+  if (int r = check_slot(s, slot)) return r;
+  Member& m = s->members[slot];
+  const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
+  HIPCHK(hipSetDevice(s->device));
+  float* dst[4] = {p, u0, u1, u2};
+  const float* src[4] = {m.dev.p, m.dev.u[0], m.dev.u[1], m.dev.u[2]};
+  for (int i = 0; i < 4; ++i)
+    if (dst[i]) HIPCHK(hipMemcpyAsync(dst[i], src[i], P * 4, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (n) *n = m.n;
+  return CST_OK;
+}
+
+int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, const float* u0,
+                         const float* u1, const float* u2) {
+  if (int r = check_slot(s, slot)) return r;
+  if (!p || !u0 || !u1 || !u2 || n < 0) return fail(CST_ERR_ARG, "null state array or negative step");
+  Member& m = s->members[slot];
+  const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
+  HIPCHK(hipSetDevice(s->device));
+  const float* src[4] = {p, u0, u1, u2};
+  float* dst[4] = {m.dev.p, m.dev.u[0], m.dev.u[1], m.dev.u[2]};
+  for (int i = 0; i < 4; ++i)
+    HIPCHK(hipMemcpyAsync(dst[i], src[i], P * 4, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  m.n = n;
+  m.primed = false;
+  return CST_OK;
+}
+
+int cst_session_set_profiling(cst_session* s, int on) {
+  if (!s) return fail(CST_ERR_ARG, "null session");
+  HIPCHK(hipSetDevice(s->device));
+  return s->prof.enable(on != 0, s->stream);
+}
+
+int cst_session_kernel_time(cst_session* s, int kind, double* total_ms, int64_t* launches) {
+  if (!s || !total_ms || !launches) return fail(CST_ERR_ARG, "null argument");
+  if (kind < 0 || kind >= Profiler::kKinds) return fail(CST_ERR_ARG, "unknown kernel kind");
+  HIPCHK(hipSetDevice(s->device));
+  if (int r = s->prof.resolve()) return r;
+  *total_ms = s->prof.ms[kind];
+  *launches = s->prof.count[kind];
+  return CST_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,34 @@
+"""Linear k-space PSTD solver, registry name ``"linear"`` (upstream ``solvers/kspace/linear.py``)."""
+
+from __future__ import annotations
+
+from typing import Any
+
+from caustica_amd.solvers.base import CWRunSpec, SolverBase, SolverCaps, SolverResult
+from caustica_amd.solvers.kspace.engine import run_cw_kspace_pstd
+from caustica_amd.solvers.registry import register
+
+
+@register
+class LinearKSpacePSTD(SolverBase):
+    """Linear full-wave k-space PSTD (1/2/3-D, CW steady state) on the hip backend."""
+
+    name = "linear"
+    caps = SolverCaps(
+        ndim=frozenset({1, 2, 3}),
+        nonlinear=False,
+        drive=frozenset({"cw"}),
+        backends=frozenset({"hip"}),
+    )
+
+    def run(self, grid, medium, source, spec: CWRunSpec | None = None, backend: str = "auto",
+            record_region=None, reference_point=None, harmonics=(1,), checkpoint=None,
+            progress=None, **kwargs: Any) -> SolverResult:
+        if kwargs:
+            raise TypeError(f"unknown run() options: {sorted(kwargs)}")
+        spec = spec or CWRunSpec()
+        self.validate(grid, medium, source)
+        return run_cw_kspace_pstd(self.name, grid, medium, source, spec, backend=backend,
+                                  record_region=record_region, reference_point=reference_point,
+                                  nonlinear=False, harmonics=harmonics, checkpoint=checkpoint,
+                                  progress=progress)

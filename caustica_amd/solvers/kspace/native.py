@@ -1,0 +1,168 @@
+"""Python handle on a libcaustica_hip session (include/caustica_hip.h).
+
+A session owns up to ``max_members`` same-shape solves on one HIP stream; a single
+solve is a one-member session. Upstream counterpart: the per-solve device state of
+``engine.py:225-278`` (maps, operators, state p/u, source, record buffers).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from caustica_amd import _native as N
+from caustica_amd.solvers.kspace.operators import MemberPlan
+
+KERNEL_KINDS = ("xgrad", "y3", "zu", "xdiv", "y1", "zp", "finalize")
+
+
+class KSpaceSession:
+    def __init__(self, padded3, max_members: int = 1, device: int = 0):
+        self.padded3 = tuple(int(v) for v in padded3)
+        self._lib = N.lib()
+        h = C.c_void_p()
+        N.check(self._lib.cst_session_create(*self.padded3, int(max_members), int(device), C.byref(h)))
+        self._h = h
+        self.max_members = int(max_members)
+        self._plans: dict[int, MemberPlan] = {}
+
+    # -------------------------------------------------------------- lifetime
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            self._lib.cst_session_destroy(self._h)
+        self._h = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        N.check(self._lib.cst_session_stream(self._h, C.byref(s)))
+        return int(s.value or 0)
+
+    # --------------------------------------------------------------- members
+    def add(self, plan: MemberPlan, max_periods: int | None = None) -> int:
+        if plan.padded3 != self.padded3:
+            raise ValueError(f"plan shape {plan.padded3} != session shape {self.padded3}")
+        d = N.MemberDesc()
+        d.dx = plan.dx
+        d.pml_vox = plan.pml_vox
+        d.pml_edge = plan.pml_edge
+        d.dt = plan.dt
+        d.c_ref = plan.c_ref
+        d.spp = plan.spp
+        d.f0 = plan.f0
+        d.amplitude = plan.amplitude
+        d.ramp_periods = plan.ramp_periods
+        keep = []
+
+        def arr(a, dt=np.float32):
+            if a is None:
+                return None
+            b = np.ascontiguousarray(a, dtype=dt)
+            keep.append(b)
+            return N.ptr(b)
+
+        d.dt_over_rho = arr(plan.dt_over_rho)
+        d.rhoc2_dt = arr(plan.rhoc2_dt)
+        d.absorb = arr(plan.absorb)
+        d.beta2_dt = arr(plan.beta2_dt)
+        d.ktab = arr(plan.ktab)
+        d.sponge = arr(plan.sponge)
+        d.n_src = int(plan.src_idx3.shape[0])
+        d.src_idx = arr(plan.src_idx3, np.int32)
+        d.src_phase = arr(plan.src_phase)
+        d.src_scale = arr(plan.src_scale)
+        for i in range(6):
+            d.conv[i] = int(plan.conv3[i])
+            d.rec[i] = int(plan.rec3[i])
+        if not 1 <= len(plan.harmonics) <= N.MAX_HARMONICS:
+            raise ValueError(f"1..{N.MAX_HARMONICS} harmonics per solve on hip")
+        d.n_harmonics = len(plan.harmonics)
+        for i, h in enumerate(plan.harmonics):
+            d.harmonics[i] = int(h)
+        if max_periods is None:
+            max_periods = plan.eff_max + 4
+        d.max_periods = int(max_periods)
+        slot = C.c_int(-1)
+        N.check(self._lib.cst_member_add(self._h, C.byref(d), C.byref(slot)))
+        self._plans[slot.value] = plan
+        return slot.value
+
+    def remove(self, slot: int) -> None:
+        N.check(self._lib.cst_member_remove(self._h, int(slot)))
+        self._plans.pop(slot, None)
+
+    def plan(self, slot: int) -> MemberPlan:
+        return self._plans[slot]
+
+    def schedule(self, slot: int, record_from: int) -> None:
+        N.check(self._lib.cst_member_schedule(self._h, int(slot), C.c_int64(int(record_from))))
+
+    def step(self, slot: int) -> int:
+        n = C.c_int64()
+        N.check(self._lib.cst_member_step(self._h, int(slot), C.byref(n)))
+        return int(n.value)
+
+    # --------------------------------------------------------------- running
+    def advance(self, slots, n_steps: int) -> None:
+        slots = [int(s) for s in slots]
+        arr = (C.c_int * len(slots))(*slots)
+        N.check(self._lib.cst_advance(self._h, len(slots), arr, int(n_steps)))
+
+    def sync(self) -> None:
+        N.check(self._lib.cst_sync(self._h))
+
+    def peaks(self, slot: int, first: int, count: int) -> np.ndarray:
+        out = np.zeros(max(count, 0), np.float32)
+        N.check(self._lib.cst_member_peaks(self._h, int(slot), int(first), int(count), N.ptr(out)))
+        return out
+
+    def result(self, slot: int, hi: int, scale: float, with_pmax: bool = True):
+        plan = self._plans[slot]
+        R = int(np.prod([plan.rec3[1] - plan.rec3[0], plan.rec3[3] - plan.rec3[2],
+                         plan.rec3[5] - plan.rec3[4]]))
+        ph = np.zeros(R, np.complex64)
+        pm = np.zeros(R, np.float32) if with_pmax else None
+        N.check(self._lib.cst_member_result(self._h, int(slot), int(hi), float(scale), N.ptr(ph),
+                                            N.ptr(pm)))
+        return ph, pm
+
+    def get_state(self, slot: int):
+        shp = self.padded3
+        arrs = [np.empty(shp, np.float32) for _ in range(4)]
+        n = C.c_int64()
+        N.check(self._lib.cst_member_get_state(self._h, int(slot), *[N.ptr(a) for a in arrs], C.byref(n)))
+        return arrs, int(n.value)
+
+    def set_state(self, slot: int, n: int, p, u0, u1, u2) -> None:
+        arrs = [np.ascontiguousarray(np.reshape(a, self.padded3), np.float32) for a in (p, u0, u1, u2)]
+        N.check(self._lib.cst_member_set_state(self._h, int(slot), C.c_int64(int(n)),
+                                               *[N.ptr(a) for a in arrs]))
+
+    # ---------------------------------------------------------------- timing
+    def set_profiling(self, on: bool) -> None:
+        N.check(self._lib.cst_session_set_profiling(self._h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        out = {}
+        for k, name in enumerate(KERNEL_KINDS):
+            ms = C.c_double()
+            cnt = C.c_int64()
+            N.check(self._lib.cst_session_kernel_time(self._h, k, C.byref(ms), C.byref(cnt)))
+            out[name] = (float(ms.value), int(cnt.value))
+        return out

@@ -1,0 +1,132 @@
+/* caustica_hip.h -- C ABI of the MI355X (gfx950) CW k-space + Pennes/CEM43 library.
+ *
+ * Plain C: pointers, sizes and status codes only. Every call returns CST_OK (0) or a
+ * nonzero status; cst_last_error() returns the message of the last failure on the
+ * calling thread. Host arrays passed in are borrowed for the duration of the call.
+ * All volumes are C-order float32 [nx][ny][nz] with z contiguous (the upstream
+ * numpy layout of the padded grid).
+ *
+ * Upstream interfaces these entry points replace (paths in the upstream caustica tree):
+ *   the backend seam       src/caustica/core/backend.py:141-190, :236-270 (get_backend -> xp, fft)
+ *   the CW solve           src/caustica/solvers/kspace/engine.py:167-586
+ *                          (setup :225-278, step() :458-482, settle loop :484-510,
+ *                           record window :522-556, result :558-586)
+ *   checkpoint state       src/caustica/solvers/kspace/engine.py:344-371
+ *   the Pennes solve       src/caustica/thermal/pennes.py:451-689 (hot loop :608-654)
+ *   CEM43 rate             src/caustica/thermal/dose.py:74-89
+ *   heat source            src/caustica/sensors.py:193-295
+ */
+#ifndef CAUSTICA_HIP_H
+#define CAUSTICA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CST_OK 0
+#define CST_ERR_ARG 1         /* invalid argument (shape, slot, size, pointer) */
+#define CST_ERR_HIP 2         /* HIP runtime failure (allocation, launch, copy) */
+#define CST_ERR_UNSUPPORTED 3 /* axis length without an instantiated kernel */
+#define CST_ERR_STATE 4       /* call out of order (e.g. member slot empty) */
+
+#define CST_MAX_BATCH 16      /* members advanced by one launch sequence */
+#define CST_MAX_HARMONICS 3
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int cst_version(void);
+/* Copy the calling thread's last error message into buf (always NUL-terminated). */
+int cst_last_error(char* buf, size_t n);
+/* Number of visible HIP devices. */
+int cst_device_count(int* n);
+/* 1 if an axis of length n is supported: axis 0/1 (x, y: complex passes) or 2 (z: real,
+ * n even). Mirrors the 2/3/5-smooth padding rule of upstream operators.py:23-39. */
+int cst_supported_length(int n, int axis);
+
+/* ------------------------------------------------------------------ CW k-space solve
+ * A session holds up to max_members solves ("members") of ONE padded shape and one HIP
+ * stream; cst_advance() steps any subset of them with one launch sequence (batched
+ * launch, blockIdx.z = member). A single solve is a session with one member.
+ * Replaces: engine.py:225-278 (backend resolve, upload of maps/operators/source). */
+typedef struct cst_session cst_session;
+
+int cst_session_create(int nx, int ny, int nz, int max_members, int device, cst_session** out);
+int cst_session_destroy(cst_session* s);
+/* The session's HIP stream (hipStream_t), for callers that order their own work. */
+int cst_session_stream(cst_session* s, void** stream);
+
+typedef struct {
+  double dx;             /* grid spacing [m] */
+  int pml_vox;           /* sponge width [voxels] (upstream grid.pml_vox) */
+  double pml_edge;       /* sponge edge factor (upstream PMLSpec.edge) */
+  double dt;             /* time step [s] (exact-period, engine.py:86-113) */
+  double c_ref;          /* kappa reference speed (c_max, engine.py:252) */
+  int spp;               /* steps per period */
+  double f0;             /* drive frequency [Hz] */
+  double amplitude;      /* drive amplitude [Pa] */
+  double ramp_periods;   /* cosine ramp length [periods] (sources.py:98-102) */
+  /* padded-shape coefficient maps (engine.py:239-249); beta2_dt NULL = linear */
+  const float* dt_over_rho;
+  const float* rhoc2_dt;
+  const float* absorb;
+  const float* beta2_dt;
+  /* float32 wavenumbers (operators.py:60-73, 90-142): kx[nx] kx_d[nx] ky[ny] ky_d[ny]
+   * kz[nz/2+1] kz_d[nz/2+1], _d with the even-axis Nyquist bin zeroed */
+  const float* ktab;
+  /* float32 sponge profiles on the padded axes (pml.py:53-73): sz[nz] sx[nx] sy[ny] */
+  const float* sponge;
+  /* source voxels (engine.py:268-278): n_src x 3 int32 indices, phase [rad],
+   * scale = float32(2 c dt / dx) at the voxel; duplicates are refused */
+  int n_src;
+  const int32_t* src_idx;
+  const float* src_phase;
+  const float* src_scale;
+  int conv[6];           /* convergence box x0,x1,y0,y1,z0,z1 (engine.py:292-296) */
+  int rec[6];            /* record box (engine.py:298-301) */
+  int n_harmonics;       /* 1..3 */
+  int harmonics[CST_MAX_HARMONICS];
+  int max_periods;       /* capacity of the per-period peak array */
+} cst_member_desc;
+
+/* Upload a member; returns its slot. Allocates its device memory (state, spectra,
+ * maps, record box). */
+int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot);
+int cst_member_remove(cst_session* s, int slot);
+/* Steps n >= record_from accumulate the phasors and p_max (engine.py:535-550). */
+int cst_member_schedule(cst_session* s, int slot, int64_t record_from);
+/* Current step counter of a member. */
+int cst_member_step(cst_session* s, int slot, int64_t* n);
+
+/* Advance every listed member by n_steps steps of engine.py:458-482, each from its own
+ * step counter, as one batched launch sequence per step (asynchronous on the session
+ * stream). Per-period max |p| over the convergence box goes to the member's peak slot
+ * floor(n / spp) during steps before record_from. */
+int cst_advance(cst_session* s, int n_slots, const int* slots, int n_steps);
+/* Block until the session stream is idle. */
+int cst_sync(cst_session* s);
+
+/* Period peaks (engine.py:490-495): out[i] = max |p| over the convergence box during
+ * period first + i (0-based). Synchronizes. NaN/inf propagate (engine.py:137-164). */
+int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out);
+/* Record-window result (engine.py:551-567): phasor of harmonic index hi as complex64
+ * (interleaved re, im) over the record box, = (Kahan sum - compensation) * scale
+ * evaluated in float64 on the device; pmax (may be NULL) = float32 max |p|. Synchronizes. */
+int cst_member_result(cst_session* s, int slot, int hi, double scale, float* phasor, float* pmax);
+/* Checkpoint state (engine.py:344-371): p, u0, u1, u2 (padded shape) and step n. */
+int cst_member_get_state(cst_session* s, int slot, float* p, float* u0, float* u1, float* u2,
+                         int64_t* n);
+int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, const float* u0,
+                         const float* u1, const float* u2);
+
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on the session
+ * stream and its duration accumulated per kernel kind. Kinds: 0 xgrad, 1 y (3 spectra),
+ * 2 zu, 3 xdiv, 4 y (1 spectrum), 5 zp, 6 record-finalize. */
+int cst_session_set_profiling(cst_session* s, int on);
+int cst_session_kernel_time(cst_session* s, int kind, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CAUSTICA_HIP_H */
