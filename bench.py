@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Benchmark of the hip CW k-space path on one MI355X.
+
+Workload (default): the 256^3 heterogeneous linear CW solve -- layered water / skin /
+fat / muscle with a seeded 2 % Gaussian random field on c and rho, dx 0.375 mm,
+16-cell sponge, bowl ROC 64 mm / aperture 64 mm at 200 kPa, CFL 0.3, 60 periods with
+the fundamental phasor from the last 2. One "step" of this benchmark is one whole
+solve (every time step of the schedule + the float64 record finalization), with the
+medium, operators and source already resident in HBM.
+
+Prints ONE JSON line: solves/s, per-solve latency percentiles, the per-pass HBM
+roofline of the step's eight kernels (algorithmic bytes / measured duration, with
+durations from HIP events on the session stream), and the CPU baseline (the float32
+NumPy/scipy.fft restatement of the upstream engine, timed on this host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def pass_bytes(padded3, nonlinear: bool) -> dict:
+    """Algorithmic HBM bytes per launch of each pass (each array read once, written once)."""
+    nx, ny, nz = padded3
+    P = nx * ny * nz
+    H = nx * ny * (nz // 2 + 1)
+    return {
+        "xgrad": 8 * H + 24 * H,                   # p-hat in, 3 gradient spectra out
+        "y3": 2 * 24 * H,                          # 3 spectra in place
+        "zu": 24 * H + 12 * P + 8 * P + 12 * P + 24 * H,  # spectra, u, dt/rho + absorb, u, u-hat
+        "xdiv": 24 * H + 8 * H,
+        "y1": 2 * 8 * H,
+        "zp": 8 * H + 4 * P + 8 * P + (4 * P if nonlinear else 0) + 4 * P + 8 * H,
+    }
+
+
+def record_extra_bytes(padded3, n_h: int) -> int:
+    """Extra bytes of a record step: Kahan sum + compensation r/w per harmonic, p_max r/w."""
+    nx, ny, nz = padded3
+    P = nx * ny * nz
+    return 32 * P * n_h + 8 * P
+
+
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*traffic*.md), if any."""
+    for f in sorted((ROOT / "profiles").glob("*traffic*.md")):
+        for line in f.read_text().splitlines():
+            m = re.match(r"\s*traffic_bytes_per_launch\s+(\S+)\s+(\S+)\s+([0-9.eE+]+)", line)
+            if m and m.group(1) == kernel and m.group(2) == workload:
+                return float(m.group(3))
+    return None
+
+
+def cpu_baseline(wl, n_steps: int, workers: int) -> dict:
+    """Median step time of the oracle's float32 restatement of the upstream engine."""
+    import numpy as np
+
+    from oracle.kspace import KSpaceOracle, RunSpec
+
+    spec = wl.spec
+    orc = KSpaceOracle(
+        shape=wl.grid.shape, dx=wl.grid.dx, pml_vox=wl.grid.pml_vox, pml_edge=wl.grid.pml.edge,
+        alpha=wl.medium.alpha, rho=wl.medium.rho, c=wl.medium.c, beta=wl.medium.beta,
+        src_idx=wl.source.indices, src_phase=wl.source.phases, amplitude=wl.source.amplitude,
+        f0=wl.source.f0, ramp_periods=wl.source.ramp_periods,
+        spec=RunSpec(**spec.model_dump()), harmonics=wl.harmonics,
+        nonlinear=wl.solver == "westervelt", precision="fp32", workers=workers,
+    )
+    orc.step(0)  # warm-up (plans, page-in)
+    times = []
+    for n in range(1, n_steps + 1):
+        t0 = time.perf_counter()
+        orc.step(n)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    steps_per_solve = wl.periods_total * orc.spp
+    del orc
+    return {
+        "value": 1.0 / (med * steps_per_solve),
+        "unit": "solves/s",
+        "cores": workers,
+        "kind": "port",
+        "sample": (f"{n_steps} timed steps (+1 warm-up) of the float32 NumPy/scipy.fft restatement "
+                   f"of the upstream step on the same medium and source, scipy.fft workers={workers}; "
+                   f"median step {med * 1e3:.1f} ms x {steps_per_solve} steps per solve"),
+        "median_step_ms": med * 1e3,
+    }
+
+
+def device_sync(sess):
+    sess.sync()
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:  # noqa: BLE001 - torch is optional plumbing here
+        pass
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed solves")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed solves")
+    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed CPU baseline steps (0: skip)")
+    ap.add_argument("--cpu-workers", type=int, default=1)
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            "bench.py runs on one GPU only: this build targets a single MI355X; independent "
+            "solves shard across GPUs with no exchange (see DESIGN.md), so no multi-GPU "
+            "figure is produced.",
+            file=sys.stderr,
+        )
+        return 2
+
+    import numpy as np
+
+    from caustica_amd import _native as N
+    from caustica_amd import workloads
+    from caustica_amd.core.backend import get_backend
+    from caustica_amd.solvers.kspace.native import KSpaceSession
+    from caustica_amd.solvers.kspace.operators import plan_member
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    get_backend("hip")
+    log(f"building workload {args.workload}")
+    wl = {"cfg0": workloads.cfg0_water_bowl, "cfg1": workloads.cfg1_hetero_linear,
+          "cfg2": workloads.cfg2_westervelt}[args.workload]()
+    plan = plan_member(wl.grid, wl.medium, wl.source, wl.spec, harmonics=wl.harmonics,
+                       reference_point=wl.reference_point, nonlinear=wl.solver == "westervelt")
+    spp = plan.spp
+    rec_steps = wl.spec.n_record_periods * spp
+    total_steps = wl.periods_total * spp
+    record_from = total_steps - rec_steps
+    sess = KSpaceSession(plan.padded3, 1)
+    slot = sess.add(plan, max_periods=wl.periods_total + 4)
+
+    def one_solve():
+        """Fresh solve of the resident member; the float64 phasor finalization runs on the
+        device, the result stays in HBM (no host copy inside the timed region)."""
+        sess.reset(slot)
+        sess.schedule(slot, record_from)
+        sess.advance([slot], total_steps)
+        for hi in range(len(plan.harmonics)):
+            N.check(N.lib().cst_member_result(sess.handle, slot, hi, 2.0 / rec_steps, None, None))
+
+    log(f"{wl.name}: spp {spp}, {total_steps} steps/solve; warmup {args.warmup}, timed {args.steps}")
+    for _ in range(args.warmup):
+        one_solve()
+    device_sync(sess)
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        one_solve()
+        device_sync(sess)
+        lat.append(time.perf_counter() - t0)
+    device_sync(sess)
+    elapsed = time.perf_counter() - t_start
+    value = args.steps / elapsed
+    peaks = sess.peaks(slot, 0, wl.periods_total - wl.spec.n_record_periods)
+    if not np.isfinite(peaks).all():
+        print("bench: non-finite field", file=sys.stderr)
+        return 3
+
+    passes = {}
+    roofline = None
+    if not args.no_profile:
+        sess.set_profiling(True)
+        one_solve()
+        device_sync(sess)
+        kt = sess.kernel_times()
+        sess.set_profiling(False)
+        nb = pass_bytes(plan.padded3, plan.beta2_dt is not None)
+        for name, (ms, cnt) in kt.items():
+            if cnt == 0 or name not in nb:
+                continue
+            avg = ms / cnt
+            gbs = nb[name] / (avg * 1e-3) / 1e9
+            passes[name] = {"launches": cnt, "avg_us": round(avg * 1e3, 2),
+                            "bytes_per_launch": nb[name], "GB_s": round(gbs, 1),
+                            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        dom = max(passes, key=lambda k: passes[k]["avg_us"] * passes[k]["launches"])
+        d = passes[dom]
+        roofline = {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": d["GB_s"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": d["frac"],
+            "traffic": read_traffic(dom, wl.name),
+        }
+        step_ms = sum(passes[k]["avg_us"] * passes[k]["launches"] for k in passes) / 1e3 / total_steps
+        step_bytes = sum(nb.values())
+        roofline["step"] = {"alg_bytes": step_bytes, "kernel_ms": round(step_ms, 4),
+                            "GB_s": round(step_bytes / (step_ms * 1e-3) / 1e9, 1)}
+
+    log(f"gpu: {value:.3f} solves/s; passes {json.dumps(passes)}")
+    cpu = None
+    if args.cpu_steps > 0:
+        log(f"cpu baseline: {args.cpu_steps} steps, workers={args.cpu_workers}")
+        cpu = cpu_baseline(wl, args.cpu_steps, args.cpu_workers)
+
+    lat_ms = sorted(x * 1e3 for x in lat)
+    out = {
+        "metric": "CW solves/s + p50/p99 latency (mixed-size queue) vs serial plain; % HBM peak",
+        "value": round(value, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered + Gaussian-random-field medium, voxelized bowl)",
+        "config": {
+            "workload": wl.name,
+            "solver": wl.solver,
+            "grid": list(wl.grid.shape),
+            "periods": wl.periods_total,
+            "spp": spp,
+            "steps_per_solve": total_steps,
+            "harmonics": list(wl.harmonics),
+        },
+        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
+                       "p99": round(float(np.percentile(lat_ms, 99)), 2)},
+        "roofline": roofline,
+        "passes": passes,
+        "cpu_baseline": cpu,
+    }
+    sess.close()
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

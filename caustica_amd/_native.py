@@ -33,6 +33,7 @@ EXPORTS = (
     "cst_session_stream",
     "cst_member_add",
     "cst_member_remove",
+    "cst_member_reset",
     "cst_member_schedule",
     "cst_member_step",
     "cst_advance",
@@ -43,6 +44,14 @@ EXPORTS = (
     "cst_member_set_state",
     "cst_session_set_profiling",
     "cst_session_kernel_time",
+    "cst_thermal_create",
+    "cst_thermal_destroy",
+    "cst_thermal_setup",
+    "cst_thermal_run",
+    "cst_thermal_status",
+    "cst_thermal_read",
+    "cst_thermal_set_profiling",
+    "cst_thermal_kernel_time",
 )
 
 
@@ -120,6 +129,7 @@ def _bind(lib):
         "cst_session_stream": ([v, C.POINTER(v)], i),
         "cst_member_add": ([v, C.POINTER(MemberDesc), C.POINTER(i)], i),
         "cst_member_remove": ([v, i], i),
+        "cst_member_reset": ([v, i], i),
         "cst_member_schedule": ([v, i, i64], i),
         "cst_member_step": ([v, i, C.POINTER(i64)], i),
         "cst_advance": ([v, i, C.POINTER(i), i], i),
@@ -130,6 +140,14 @@ def _bind(lib):
         "cst_member_set_state": ([v, i, i64, p, p, p, p], i),
         "cst_session_set_profiling": ([v, i], i),
         "cst_session_kernel_time": ([v, i, C.POINTER(d), C.POINTER(i64)], i),
+        "cst_thermal_create": ([i, i, i, i, C.POINTER(v)], i),
+        "cst_thermal_destroy": ([v], i),
+        "cst_thermal_setup": ([v, C.POINTER(ThermalDesc)], i),
+        "cst_thermal_run": ([v, i], i),
+        "cst_thermal_status": ([v, C.POINTER(i)], i),
+        "cst_thermal_read": ([v, p, p, p], i),
+        "cst_thermal_set_profiling": ([v, i], i),
+        "cst_thermal_kernel_time": ([v, C.POINTER(d), C.POINTER(i64)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

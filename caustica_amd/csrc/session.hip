@@ -380,6 +380,28 @@ int cst_member_remove(cst_session* s, int slot) {
   return CST_OK;
 }
 
+int cst_member_reset(cst_session* s, int slot) {
+  if (int r = check_slot(s, slot)) return r;
+  Member& m = s->members[slot];
+  const Shape& sh = s->s;
+  const int64_t P = (int64_t)sh.nx * sh.ny * sh.nz;
+  const int64_t H = (int64_t)sh.nx * sh.ny * sh.nzh;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipMemsetAsync(m.dev.p, 0, 4 * P * 4, s->stream));
+  HIPCHK(hipMemsetAsync(m.dev.sp, 0, 4 * H * 8, s->stream));
+  for (int h = 0; h < m.nh; ++h) {
+    HIPCHK(hipMemsetAsync(m.dev.acc[h], 0, m.rec_count * 8, s->stream));
+    HIPCHK(hipMemsetAsync(m.dev.comp[h], 0, m.rec_count * 8, s->stream));
+    m.finalized[h] = false;
+  }
+  HIPCHK(hipMemsetAsync(m.dev.pmax, 0, m.rec_count * 4, s->stream));
+  HIPCHK(hipMemsetAsync(m.dev.peaks, 0, (size_t)m.max_periods * 4, s->stream));
+  m.n = 0;
+  m.record_from = INT64_MAX;
+  m.primed = true;
+  return CST_OK;
+}
+
 int cst_member_schedule(cst_session* s, int slot, int64_t record_from) {
   if (int r = check_slot(s, slot)) return r;
   s->members[slot].record_from = record_from;

@@ -1,0 +1,186 @@
+"""Acoustic -> thermal join: volumetric heat deposition Q from a CW result.
+
+Upstream ``src/caustica/sensors.py:100-435``. For a locally progressive wave the
+absorbed power density of harmonic n is Q_n = 2 alpha_n I_n = alpha_n |P_n|^2 / (rho c)
+(P a peak amplitude, p(t) = Re{P exp(-i omega t)}). Harmonics above f0 need a stated
+absorption model (``alpha_power_law_y`` -> alpha_n = alpha_f0 n^y, or explicit
+``alpha_at_harmonics``); a result with harmonics and no model is refused, as upstream.
+The sum is formed in float64 and stored float32.
+"""
+
+from __future__ import annotations
+
+import warnings
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+from caustica_amd.core.backend import CausticaWarning
+
+ALPHA_MODEL_F0 = "f0_only"
+ALPHA_MODEL_POWER_LAW = "power_law"
+ALPHA_MODEL_EXPLICIT = "explicit"
+ALPHA_POWER_LAW_Y_RANGE = (0.0, 3.0)
+
+
+def _as_map(value: Any, shape: tuple, what: str) -> np.ndarray:
+    arr = np.asarray(value, dtype=np.float64)
+    if arr.shape not in ((), shape):
+        raise ValueError(f"{what} has shape {arr.shape}, neither a scalar nor the field shape {shape}.")
+    if not np.isfinite(arr).all():
+        raise ValueError(f"{what} contains non-finite values.")
+    return np.broadcast_to(arr, shape)
+
+
+@dataclass(frozen=True, eq=False)
+class HeatingSource:
+    q: np.ndarray
+    region: tuple
+    dx: float
+    harmonics: tuple
+    alpha_model: str
+    meta: dict[str, Any] = field(default_factory=dict)
+
+    @classmethod
+    def from_result(cls, result, medium, dx: float, *, harmonics=None, alpha_power_law_y=None,
+                    alpha_at_harmonics=None) -> HeatingSource:
+        phasors = dict(result.phasors) if result.phasors else {1: result.phasor}
+        region = tuple(result.region)
+        if len(region) != medium.alpha.ndim:
+            raise ValueError(f"result.region has rank {len(region)} but the medium is "
+                             f"{medium.alpha.ndim}-D -- they are not the same grid.")
+        shape = tuple(np.shape(phasors[next(iter(phasors))]))
+        maps = {k: getattr(medium, k)[region] for k in ("alpha", "rho", "c")}
+        for name, vol in maps.items():
+            if vol.shape != shape:
+                raise ValueError(f"medium.{name} sliced by result.region has shape {vol.shape} but "
+                                 f"the recorded phasor has shape {shape}.")
+        return cls.from_phasors(phasors, alpha=maps["alpha"], rho=maps["rho"], c=maps["c"], dx=dx,
+                                region=region, harmonics=harmonics,
+                                alpha_power_law_y=alpha_power_law_y,
+                                alpha_at_harmonics=alpha_at_harmonics)
+
+    @classmethod
+    def from_phasors(cls, phasors: dict, *, alpha, rho, c, dx: float, region=None, harmonics=None,
+                     alpha_power_law_y=None, alpha_at_harmonics=None) -> HeatingSource:
+        if not phasors:
+            raise ValueError("phasors is empty: there is no field to turn into heat.")
+        if not (np.isfinite(dx) and dx > 0):
+            raise ValueError(f"dx must be a positive finite spacing in meters, got {dx}")
+        available = tuple(sorted(int(n) for n in phasors))
+        if harmonics is None:
+            requested = available
+        else:
+            requested = tuple(dict.fromkeys(int(n) for n in harmonics))
+            missing = [n for n in requested if n not in phasors]
+            if missing:
+                raise ValueError(f"harmonics {missing} were asked for but are not in the result "
+                                 f"(available: {list(available)}).")
+        if any(n < 1 for n in requested):
+            raise ValueError(f"harmonic numbers must be >= 1, got {requested}")
+        shape = tuple(np.shape(phasors[requested[0]]))
+        for n in requested:
+            if tuple(np.shape(phasors[n])) != shape:
+                raise ValueError(f"harmonic {n} has shape {np.shape(phasors[n])}, not {shape}.")
+        alpha_f0 = _as_map(alpha, shape, "alpha")
+        rho_m = _as_map(rho, shape, "rho")
+        c_m = _as_map(c, shape, "c")
+        if (alpha_f0 < 0).any():
+            raise ValueError("alpha must be >= 0 Np/m (absorption cannot be negative).")
+        if (rho_m <= 0).any() or (c_m <= 0).any():
+            raise ValueError("rho and c must be > 0 everywhere to convert |P|^2 into intensity.")
+        alpha_by_h = _harmonic_alpha(requested, alpha_f0, available, alpha_power_law_y, alpha_at_harmonics)
+        if alpha_at_harmonics is not None:
+            model = ALPHA_MODEL_EXPLICIT
+        elif alpha_power_law_y is not None and max(requested) > 1:
+            model = ALPHA_MODEL_POWER_LAW
+        else:
+            model = ALPHA_MODEL_F0
+        inv_rho_c = 1.0 / (rho_m * c_m)
+        q = np.zeros(shape, np.float64)
+        mean_by_h = {}
+        for n in requested:
+            p = np.asarray(phasors[n])
+            if not np.isfinite(p).all():
+                raise ValueError(f"the phasor of harmonic {n} contains non-finite values.")
+            contrib = alpha_by_h[n] * (np.abs(p).astype(np.float64) ** 2) * inv_rho_c
+            q += contrib
+            mean_by_h[n] = float(contrib.mean())
+        if region is None:
+            region = tuple(slice(0, n) for n in shape)
+        meta = {
+            "alpha_model": model,
+            "alpha_power_law_y": alpha_power_law_y,
+            "mean_q_by_harmonic_w_m3": mean_by_h,
+            "harmonics_available": list(available),
+            "intensity_model": "local plane wave: I = |P|^2 / (2 rho c)",
+        }
+        return cls(q=np.ascontiguousarray(q, dtype=np.float32), region=tuple(region), dx=float(dx),
+                   harmonics=requested, alpha_model=model, meta=meta)
+
+    @property
+    def shape(self):
+        return tuple(self.q.shape)
+
+    @property
+    def q_max(self) -> float:
+        return float(self.q.max())
+
+    @property
+    def total_power_w(self) -> float:
+        return float(self.q.sum()) * self.dx ** self.q.ndim
+
+    def scaled(self, factor: float) -> HeatingSource:
+        """Same distribution, Q multiplied by ``factor`` (drive scaling: Q goes as p^2)."""
+        return HeatingSource(q=(self.q.astype(np.float64) * factor).astype(np.float32),
+                             region=self.region, dx=self.dx, harmonics=self.harmonics,
+                             alpha_model=self.alpha_model, meta={**self.meta, "scaled_by": factor})
+
+    def embed(self, shape) -> np.ndarray:
+        shape = tuple(int(s) for s in shape)
+        if len(shape) != self.q.ndim:
+            raise ValueError(f"shape {shape} has rank {len(shape)}, need {self.q.ndim}")
+        full = np.zeros(shape, np.float32)
+        try:
+            full[self.region] = self.q
+        except ValueError as exc:
+            raise ValueError(f"the region {self.region} does not fit a grid of shape {shape}: {exc}") from None
+        return full
+
+
+def _harmonic_alpha(requested, alpha_f0, available, y, explicit) -> dict:
+    if y is not None and explicit is not None:
+        raise ValueError("pass EITHER alpha_power_law_y OR alpha_at_harmonics, not both.")
+    high = [n for n in requested if n >= 2]
+    if high and y is None and explicit is None:
+        raise ValueError(
+            f"this result carries harmonics {high} above the fundamental, but alpha is only known "
+            f"at f0. State the model: alpha_power_law_y=y (alpha_n = alpha_f0 n^y), "
+            f"alpha_at_harmonics={{n: alpha_np_m}}, or harmonics=(1,) for fundamental only."
+        )
+    if y is not None:
+        lo, hi = ALPHA_POWER_LAW_Y_RANGE
+        if not (np.isfinite(y) and lo <= float(y) <= hi):
+            raise ValueError(f"alpha_power_law_y={y} is outside the physical range [{lo}, {hi}].")
+        return {n: alpha_f0 * float(n) ** float(y) for n in requested}
+    if explicit is not None:
+        keys = {int(n): v for n, v in explicit.items()}
+        missing = [n for n in requested if n >= 2 and n not in keys]
+        if missing:
+            raise ValueError(f"alpha_at_harmonics is missing harmonic(s) {missing}.")
+        out = {}
+        hetero = float(alpha_f0.min()) != float(alpha_f0.max())
+        for n in requested:
+            if n not in keys:
+                out[n] = alpha_f0
+                continue
+            if hetero and np.asarray(keys[n]).ndim == 0:
+                warnings.warn(f"alpha_at_harmonics[{n}] is a scalar but alpha at f0 is heterogeneous; "
+                              f"the scalar is applied to every voxel.", CausticaWarning, stacklevel=4)
+            arr = _as_map(keys[n], alpha_f0.shape, f"alpha_at_harmonics[{n}]")
+            if (arr < 0).any():
+                raise ValueError(f"alpha_at_harmonics[{n}] has negative values.")
+            out[n] = arr
+        return out
+    return {n: alpha_f0 for n in requested}

@@ -107,6 +107,9 @@ class KSpaceSession:
         N.check(self._lib.cst_member_remove(self._h, int(slot)))
         self._plans.pop(slot, None)
 
+    def reset(self, slot: int) -> None:
+        N.check(self._lib.cst_member_reset(self._h, int(slot)))
+
     def plan(self, slot: int) -> MemberPlan:
         return self._plans[slot]
 

@@ -112,8 +112,11 @@ def cfg0_water_bowl() -> Workload:
     full = np.ones(shape, np.float32)
     medium = Medium(alpha=full * alpha, rho=full * 1000.0, c=full * 1500.0, beta=full * 0.0)
     src = bowl_cw_source(grid, F0, 1.0e5, aperture_radius=8e-3, roc=20e-3, apex_vox=(32, 32, 10))
-    spec = pinned_spec(grid, medium, src, cfl=0.3, periods_total=40, n_record=4)
-    return Workload("cfg0_water_bowl_64x64x96", "linear", grid, medium, src, spec, (1,), 40)
+    focus = (32, 32, 10 + grid.voxels_for(20e-3))
+    spec = pinned_spec(grid, medium, src, cfl=0.3, periods_total=40, n_record=4,
+                       reference_point=focus)
+    return Workload("cfg0_water_bowl_64x64x96", "linear", grid, medium, src, spec, (1,), 40,
+                    reference_point=focus)
 
 
 def cfg1_hetero_linear(seed: int = 1, shape=(256, 256, 256), steer_vox=None) -> Workload:
@@ -123,15 +126,17 @@ def cfg1_hetero_linear(seed: int = 1, shape=(256, 256, 256), steer_vox=None) -> 
     grid = Grid(shape, dx, PMLSpec(thickness=16 * dx))
     medium = layered_medium(shape, dx, seed=seed)
     apex = (shape[0] // 2, shape[1] // 2, grid.pml_vox)
+    focus = (apex[0], apex[1], apex[2] + grid.voxels_for(64e-3))
     phases = None
     if steer_vox is not None:
-        target = tuple(np.asarray(apex) + np.array([0, 0, round(64e-3 / dx)]) + np.asarray(steer_vox))
-        phases = lambda idx: focus_phases(idx, dx, target, 1500.0, F0)  # noqa: E731
+        focus = tuple(int(v) for v in np.asarray(focus) + np.asarray(steer_vox))
+        phases = lambda idx: focus_phases(idx, dx, focus, 1500.0, F0)  # noqa: E731
     src = bowl_cw_source(grid, F0, 2.0e5, aperture_radius=32e-3, roc=64e-3, apex_vox=apex,
                          phases=phases)
-    spec = pinned_spec(grid, medium, src, cfl=0.3, periods_total=60, n_record=2)
+    spec = pinned_spec(grid, medium, src, cfl=0.3, periods_total=60, n_record=2,
+                       reference_point=focus)
     return Workload(f"cfg1_hetero_linear_{'x'.join(map(str, shape))}", "linear", grid, medium,
-                    src, spec, (1,), 60)
+                    src, spec, (1,), 60, reference_point=focus)
 
 
 def cfg2_westervelt(seed: int = 2, shape=(256, 256, 384)) -> Workload:
@@ -143,6 +148,8 @@ def cfg2_westervelt(seed: int = 2, shape=(256, 256, 384)) -> Workload:
     medium = layered_medium(shape, dx, seed=seed, nonlinear=True)
     apex = (shape[0] // 2, shape[1] // 2, grid.pml_vox)
     src = bowl_cw_source(grid, F0, 1.0e6, aperture_radius=31.75e-3, roc=64e-3, apex_vox=apex)
-    spec = pinned_spec(grid, medium, src, cfl=0.2, periods_total=80, n_record=2)
+    focus = (apex[0], apex[1], apex[2] + grid.voxels_for(64e-3))
+    spec = pinned_spec(grid, medium, src, cfl=0.2, periods_total=80, n_record=2,
+                       reference_point=focus)
     return Workload(f"cfg2_westervelt_{'x'.join(map(str, shape))}", "westervelt", grid, medium,
-                    src, spec, (1, 2, 3), 80)
+                    src, spec, (1, 2, 3), 80, reference_point=focus)

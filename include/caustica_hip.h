@@ -94,6 +94,9 @@ typedef struct {
  * maps, record box). */
 int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot);
 int cst_member_remove(cst_session* s, int slot);
+/* Zero the member's state, spectra, record sums and peaks and set its step counter to
+ * 0, keeping its uploaded maps and source (a fresh solve of the same setup). */
+int cst_member_reset(cst_session* s, int slot);
 /* Steps n >= record_from accumulate the phasors and p_max (engine.py:535-550). */
 int cst_member_schedule(cst_session* s, int slot, int64_t record_from);
 /* Current step counter of a member. */
@@ -125,6 +128,41 @@ int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, co
  * 2 zu, 3 xdiv, 4 y (1 spectrum), 5 zp, 6 record-finalize. */
 int cst_session_set_profiling(cst_session* s, int on);
 int cst_session_kernel_time(cst_session* s, int kind, double* total_ms, int64_t* launches);
+
+/* ------------------------------------------------------------------ Pennes + CEM43
+ * One thermal solve on a grid nx x ny x nz; state theta = T - T_ref (float32), CEM43 dose
+ * summed in float64. Replaces the hot loop of thermal/pennes.py:608-654 (coefficients
+ * :404-447 are assembled by the caller, as upstream does before the loop). */
+typedef struct cst_thermal cst_thermal;
+
+typedef struct {
+  double dx;            /* grid spacing [m] */
+  const float* k;       /* conductivity [W/m/K] (faces use the harmonic mean, pennes.py:404-412) */
+  const float* perf;    /* w_b rho_b C_b [W/m^3/K] (pennes.py:440-443) */
+  const float* coeff;   /* dt_sub / (rho C) [m^3 K / J] (pennes.py:561) */
+  const float* q;       /* heat source [W/m^3] or NULL */
+  const float* theta0;  /* initial T - T_ref */
+  const float* dose0;   /* CEM43 already accumulated [min] or NULL */
+  int boundary;         /* Dirichlet wall mask: bit 0 x, bit 1 y, bit 2 z walls held at
+                         * theta_wall; 0 = insulated everywhere */
+  float theta_wall;     /* wall temperature - T_ref */
+  float t_ref;          /* reference temperature [C] (dose uses T = theta + t_ref) */
+  double half_dt_min;   /* 0.5 * dt_sub / 60 (trapezoid weight, minutes) */
+  int dose;             /* accumulate CEM43 */
+} cst_thermal_desc;
+
+int cst_thermal_create(int nx, int ny, int nz, int device, cst_thermal** out);
+int cst_thermal_destroy(cst_thermal* t);
+/* Upload coefficients and the initial state; theta_max starts at theta0. */
+int cst_thermal_setup(cst_thermal* t, const cst_thermal_desc* d);
+/* Run n_substeps explicit sub-steps (asynchronous on the thermal stream). */
+int cst_thermal_run(cst_thermal* t, int n_substeps);
+/* 1 if any theta or dose value has stopped being finite (pennes.py:756-781). Synchronizes. */
+int cst_thermal_status(cst_thermal* t, int* nonfinite);
+/* Copy theta, theta_max (float32) and the dose (float64 sum rounded to float32); any may be NULL. */
+int cst_thermal_read(cst_thermal* t, float* theta, float* theta_max, float* dose);
+int cst_thermal_set_profiling(cst_thermal* t, int on);
+int cst_thermal_kernel_time(cst_thermal* t, double* total_ms, int64_t* launches);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,58 @@
+"""Parity of the HIP Pennes + CEM43 path against the CPU oracle and the upstream fixtures.
+
+Tolerances: temperature and temperature_max <= 1e-3 K absolute, CEM43 <= 1e-3 relative
+(to the map's maximum) against the float64 oracle; the upstream float32 fixture agrees
+with the oracle at ~1e-5 K, so the same bars hold against it.
+"""
+
+import numpy as np
+import pytest
+
+from oracle.pennes import PennesOracle
+from tests._golden import load
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["pennes_insulated", "pennes_dirichlet_substep"])
+def test_pennes_matches_oracle_and_upstream(name):
+    from caustica_amd.thermal import PennesSolver, ThermalMedium, ThermalResult
+
+    print(f"[pennes] {name}", flush=True)
+    g = load(name)
+    m = g["meta"]
+    med = ThermalMedium(k=g["k"], rho=g["rho"], specific_heat=g["specific_heat"],
+                        perfusion=g["perfusion"], dx=float(g["dx"]))
+    solver = PennesSolver("hip", boundary=m["boundary"], on_unstable=m["on_unstable"])
+    dt = float(g["dt"])
+    hot = solver.solve(g["temperature0"], g["q"], med, dt=dt, n_steps=m["n_hot"], dose=True)
+    cool = solver.solve(hot.temperature, None, med, dt=dt, n_steps=m["n_cool"], dose=True,
+                        dose0=hot.dose_cem43)
+    chained = ThermalResult.chain([hot, cool])
+    assert hot.substeps == int(g["substeps"])
+    assert hot.meta["backend"] == "hip"
+    o = PennesOracle(g["k"], g["rho"], g["specific_heat"], g["perfusion"], float(g["dx"]),
+                     boundary=m["boundary"], on_unstable=m["on_unstable"], precision="fp64")
+    oh = o.solve(g["temperature0"], g["q"], dt, m["n_hot"], dose=True)
+    oc = o.solve(oh.temperature.astype(np.float32), None, dt, m["n_cool"], dose=True,
+                 dose0=oh.dose.astype(np.float32))
+    for dev, ref in ((hot.temperature, oh.temperature), (hot.temperature_max, oh.temperature_max),
+                     (cool.temperature, oc.temperature), (hot.temperature, g["hot_temperature"]),
+                     (chained.temperature_max, g["chained_temperature_max"])):
+        assert np.abs(dev - ref).max() < 1e-3
+    for dev, ref in ((hot.dose_cem43, oh.dose), (cool.dose_cem43, oc.dose),
+                     (cool.dose_cem43, g["cool_dose"])):
+        assert np.abs(dev - ref).max() <= 1e-3 * np.abs(ref).max()
+
+
+def test_pennes_finite_guard_raises():
+    from caustica_amd.thermal import PennesSolver, ThermalDivergedError, ThermalMedium
+
+    shape = (8, 8, 8)
+    med = ThermalMedium(k=np.full(shape, 0.5), rho=np.full(shape, 1000.0),
+                        specific_heat=np.full(shape, 3600.0), perfusion=np.zeros(shape), dx=1e-3)
+    q = np.zeros(shape, np.float32)
+    q[4, 4, 4] = 1e38  # overflows float32 within one step
+    with pytest.raises(ThermalDivergedError):
+        PennesSolver("hip", finite_check_every=1).solve(np.full(shape, 37.0), q, med, dt=0.1,
+                                                        n_steps=3, dose=True)
