@@ -150,7 +150,11 @@ def main() -> int:
     rec_steps = wl.spec.n_record_periods * spp
     total_steps = wl.periods_total * spp
     record_from = total_steps - rec_steps
-    sess = KSpaceSession(plan.padded3, 1)
+    sess = KSpaceSession(plan.padded3, 2)
+    from caustica_amd.tune import autotune
+
+    choice = autotune(sess, plan, use_cache=False)
+    log(f"autotuned variants {choice}")
     slot = sess.add(plan, max_periods=wl.periods_total + 4)
 
     def one_solve():
@@ -210,7 +214,7 @@ def main() -> int:
             "traffic": read_traffic(dom, wl.name),
         }
         step_ms = sum(passes[k]["avg_us"] * passes[k]["launches"] for k in passes) / 1e3 / total_steps
-        step_bytes = sum(nb.values())
+        step_bytes = sum(nb[k] * passes[k]["launches"] for k in passes) / total_steps
         roofline["step"] = {"alg_bytes": step_bytes, "kernel_ms": round(step_ms, 4),
                             "GB_s": round(step_bytes / (step_ms * 1e-3) / 1e9, 1)}
 
@@ -243,6 +247,7 @@ def main() -> int:
             "steps_per_solve": total_steps,
             "harmonics": list(wl.harmonics),
         },
+        "variants": {str(k): v for k, v in choice.items()},
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
                        "p99": round(float(np.percentile(lat_ms, 99)), 2)},
         "roofline": roofline,

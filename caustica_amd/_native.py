@@ -34,6 +34,9 @@ EXPORTS = (
     "cst_member_add",
     "cst_member_remove",
     "cst_member_reset",
+    "cst_session_trim",
+    "cst_session_pooled_bytes",
+    "cst_device_memory",
     "cst_member_schedule",
     "cst_member_step",
     "cst_advance",
@@ -42,6 +45,7 @@ EXPORTS = (
     "cst_member_result",
     "cst_member_get_state",
     "cst_member_set_state",
+    "cst_session_set_variant",
     "cst_session_set_profiling",
     "cst_session_kernel_time",
     "cst_thermal_create",
@@ -130,6 +134,9 @@ def _bind(lib):
         "cst_member_add": ([v, C.POINTER(MemberDesc), C.POINTER(i)], i),
         "cst_member_remove": ([v, i], i),
         "cst_member_reset": ([v, i], i),
+        "cst_session_trim": ([v, C.POINTER(C.c_size_t)], i),
+        "cst_session_pooled_bytes": ([v, C.POINTER(C.c_size_t)], i),
+        "cst_device_memory": ([i, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], i),
         "cst_member_schedule": ([v, i, i64], i),
         "cst_member_step": ([v, i, C.POINTER(i64)], i),
         "cst_advance": ([v, i, C.POINTER(i), i], i),
@@ -138,6 +145,7 @@ def _bind(lib):
         "cst_member_result": ([v, i, i, d, p, p], i),
         "cst_member_get_state": ([v, i, p, p, p, p, C.POINTER(i64)], i),
         "cst_member_set_state": ([v, i, i64, p, p, p, p], i),
+        "cst_session_set_variant": ([v, i, i], i),
         "cst_session_set_profiling": ([v, i], i),
         "cst_session_kernel_time": ([v, i, C.POINTER(d), C.POINTER(i64)], i),
         "cst_thermal_create": ([i, i, i, i, C.POINTER(v)], i),
@@ -185,6 +193,12 @@ def check(code: int) -> None:
 
 def ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_memory(device: int = 0) -> tuple[int, int]:
+    free, total = C.c_size_t(0), C.c_size_t(0)
+    check(lib().cst_device_memory(int(device), C.byref(free), C.byref(total)))
+    return int(free.value), int(total.value)
 
 
 def device_count() -> int:

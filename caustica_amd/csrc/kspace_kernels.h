@@ -11,8 +11,9 @@
 //   zp     z-C2R of sp -> div u; p update (+ Westervelt), absorb, sponge, source, peak,
 //          record; z-R2C of p -> sp
 //   y fwd  (1 spectrum)
-// Every line FFT is a two-stage Stockham-style split L = A * B across T threads of a
-// line, with the exchange and the twiddle table in LDS (see Plan<L>).
+// Every line FFT is a two-stage split L = A * B across T threads of a line, with the
+// exchange and the twiddle table in LDS. Spectrum rows are padded to a multiple of 16
+// bins (128 bytes) so that every column tile of the strided passes starts on a cache line.
 #pragma once
 #include <utility>
 
@@ -33,28 +34,40 @@ constexpr int pow2_floor(int v) {
   return p;
 }
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
-// L = A * B; step 1: A DFTs of size B; step 2: B DFTs of size A; T threads per line.
+// L = A * B; stage 1: A DFTs of size B; stage 2: B DFTs of size A.
 template <int L>
 struct Plan {
   static constexpr int A = plan_a(L);
   static constexpr int B = L / A;
   static constexpr int G = cgcd(A, B);
-  static constexpr int T = G >= 4 ? G : B;
-  static constexpr int R1 = (A + T - 1) / T;
-  static constexpr int R2 = (B + T - 1) / T;
+  static constexpr int T = G >= 4 ? G : B;  // threads per line in the strided passes
   // strided passes: columns per block (tile of L x TC complex in LDS, <= 48 KiB)
-  static constexpr int TC = pow2_floor(cmin(256 / T, 6144 / L)) < 1 ? 1 : pow2_floor(cmin(256 / T, 6144 / L));
+  static constexpr int TC = cmax(1, pow2_floor(cmin(256 / T, 6144 / L)));
 };
 
-// z passes: complex length M = nz / 2, LPB lines per block
-template <int M>
+// Distribution of one line FFT over T threads.
+template <int L, int T>
+struct Split {
+  static constexpr int A = Plan<L>::A;
+  static constexpr int B = Plan<L>::B;
+  static constexpr int R1 = (A + T - 1) / T;  // stage-1 DFTs per thread
+  static constexpr int R2 = (B + T - 1) / T;  // stage-2 DFTs per thread
+};
+
+// z passes: complex length M = nz / 2; ZT threads per line, LPB lines per block.
+// Variant V = 0: ZT = max(A, B) (every lane busy in the larger stage); V = 1: twice the
+// threads per line (half the registers per thread). The session picks one per shape.
+template <int M, int V>
 struct ZPlan {
-  using P = Plan<M>;
-  static constexpr int LS = M + 2;  // per-line LDS stride (float2), holds M + 1 spectrum bins
-  static constexpr int LPB_raw = pow2_floor(cmin(256 / P::T, 5120 / LS));
-  static constexpr int LPB = LPB_raw < 1 ? 1 : LPB_raw;
-  static constexpr int THREADS = LPB * P::T;
+  static constexpr int A = Plan<M>::A;
+  static constexpr int B = Plan<M>::B;
+  static constexpr int ZT = cmin(64, (V + 1) * cmax(A, B));
+  static constexpr int LSA = M + 2;                 // natural-order line buffer (M + 1 bins)
+  static constexpr int LSE = M + (M + A - 1) / A;   // exchange buffer, padded every A rows
+  static constexpr int LPB = cmax(1, pow2_floor(cmin((V + 1) * 256 / ZT, 4480 / (LSA + LSE))));
+  static constexpr int THREADS = LPB * ZT;
 };
 
 template <int DIR>
@@ -64,75 +77,76 @@ __device__ __forceinline__ float2 twmul(float2 v, float2 w) {
 
 // ---------------------------------------------------------------- stages
 // stage "in": natural-order input x[n1 + A n2]  ->  X[k2 + B k1] held as X[r][k1], k2 = t + T r
-template <int L, class Load>
-__device__ __forceinline__ void in_load(int t, Load load, float2 (&V)[Plan<L>::R1][Plan<L>::B]) {
-  using P = Plan<L>;
+template <int L, int T, class Load>
+__device__ __forceinline__ void in_load(int t, Load load, float2 (&V)[Split<L, T>::R1][Split<L, T>::B]) {
+  using S = Split<L, T>;
 #pragma unroll
-  for (int r = 0; r < P::R1; ++r) {
-    const int n1 = t + P::T * r;
+  for (int r = 0; r < S::R1; ++r) {
+    const int n1 = t + T * r;
 #pragma unroll
-    for (int n2 = 0; n2 < P::B; ++n2) V[r][n2] = (n1 < P::A) ? load(n1 + P::A * n2) : make_float2(0.f, 0.f);
+    for (int n2 = 0; n2 < S::B; ++n2)
+      V[r][n2] = (n1 < S::A) ? load(n1 + S::A * n2) : make_float2(0.f, 0.f);
   }
 }
 
-template <int L, int DIR, class At>
-__device__ __forceinline__ void in_compute(int t, float2 (&V)[Plan<L>::R1][Plan<L>::B], At at,
+template <int L, int T, int DIR, class At>
+__device__ __forceinline__ void in_compute(int t, float2 (&V)[Split<L, T>::R1][Split<L, T>::B], At at,
                                            const float2* tw) {
-  using P = Plan<L>;
+  using S = Split<L, T>;
 #pragma unroll
-  for (int r = 0; r < P::R1; ++r) {
-    const int n1 = t + P::T * r;
-    if (n1 < P::A) {
-      Dft<P::B, DIR>::run(V[r]);
+  for (int r = 0; r < S::R1; ++r) {
+    const int n1 = t + T * r;
+    if (n1 < S::A) {
+      Dft<S::B, DIR>::run(V[r]);
 #pragma unroll
-      for (int k2 = 0; k2 < P::B; ++k2) *at(n1 + P::A * k2) = twmul<DIR>(V[r][k2], tw[n1 * k2]);
+      for (int k2 = 0; k2 < S::B; ++k2) *at(n1 + S::A * k2) = twmul<DIR>(V[r][k2], tw[n1 * k2]);
     }
   }
 }
 
-template <int L, int DIR, class At>
-__device__ __forceinline__ void in_finish(int t, At at, float2 (&X)[Plan<L>::R2][Plan<L>::A]) {
-  using P = Plan<L>;
+template <int L, int T, int DIR, class At>
+__device__ __forceinline__ void in_finish(int t, At at, float2 (&X)[Split<L, T>::R2][Split<L, T>::A]) {
+  using S = Split<L, T>;
 #pragma unroll
-  for (int r = 0; r < P::R2; ++r) {
-    const int k2 = t + P::T * r;
-    if (k2 < P::B) {
+  for (int r = 0; r < S::R2; ++r) {
+    const int k2 = t + T * r;
+    if (k2 < S::B) {
 #pragma unroll
-      for (int n1 = 0; n1 < P::A; ++n1) X[r][n1] = *at(n1 + P::A * k2);
-      Dft<P::A, DIR>::run(X[r]);
+      for (int n1 = 0; n1 < S::A; ++n1) X[r][n1] = *at(n1 + S::A * k2);
+      Dft<S::A, DIR>::run(X[r]);
     }
   }
 }
 
 // stage "out": input X[k2 + B k1] as X[r][k1]  ->  natural x[m2 + A j] as V[r][j], m2 = t + T r
-template <int L, int DIR, class At>
-__device__ __forceinline__ void out_compute(int t, float2 (&X)[Plan<L>::R2][Plan<L>::A], At at,
+template <int L, int T, int DIR, class At>
+__device__ __forceinline__ void out_compute(int t, float2 (&X)[Split<L, T>::R2][Split<L, T>::A], At at,
                                             const float2* tw) {
-  using P = Plan<L>;
+  using S = Split<L, T>;
 #pragma unroll
-  for (int r = 0; r < P::R2; ++r) {
-    const int k2 = t + P::T * r;
-    if (k2 < P::B) {
-      float2 v[P::A];
+  for (int r = 0; r < S::R2; ++r) {
+    const int k2 = t + T * r;
+    if (k2 < S::B) {
+      float2 v[S::A];
 #pragma unroll
-      for (int i = 0; i < P::A; ++i) v[i] = X[r][i];
-      Dft<P::A, DIR>::run(v);
+      for (int i = 0; i < S::A; ++i) v[i] = X[r][i];
+      Dft<S::A, DIR>::run(v);
 #pragma unroll
-      for (int m2 = 0; m2 < P::A; ++m2) *at(k2 + P::B * m2) = twmul<DIR>(v[m2], tw[k2 * m2]);
+      for (int m2 = 0; m2 < S::A; ++m2) *at(k2 + S::B * m2) = twmul<DIR>(v[m2], tw[k2 * m2]);
     }
   }
 }
 
-template <int L, int DIR, class At>
-__device__ __forceinline__ void out_finish(int t, At at, float2 (&V)[Plan<L>::R1][Plan<L>::B]) {
-  using P = Plan<L>;
+template <int L, int T, int DIR, class At>
+__device__ __forceinline__ void out_finish(int t, At at, float2 (&V)[Split<L, T>::R1][Split<L, T>::B]) {
+  using S = Split<L, T>;
 #pragma unroll
-  for (int r = 0; r < P::R1; ++r) {
-    const int m2 = t + P::T * r;
-    if (m2 < P::A) {
+  for (int r = 0; r < S::R1; ++r) {
+    const int m2 = t + T * r;
+    if (m2 < S::A) {
 #pragma unroll
-      for (int k2 = 0; k2 < P::B; ++k2) V[r][k2] = *at(k2 + P::B * m2);
-      Dft<P::B, DIR>::run(V[r]);
+      for (int k2 = 0; k2 < S::B; ++k2) V[r][k2] = *at(k2 + S::B * m2);
+      Dft<S::B, DIR>::run(V[r]);
     }
   }
 }
@@ -149,76 +163,97 @@ struct LineAt {
   __device__ __forceinline__ float2* operator()(int row) const { return base + row; }
 };
 
-__device__ __forceinline__ float sincf_half(float x) {  // sin(x) / x
-  return x == 0.f ? 1.f : sinf(x) / x;
+// Line exchange buffer with one padding slot after every PAD rows: stage 2 of the split
+// reads rows n1 + A k2 at stride A across threads, which would otherwise hit the same
+// LDS banks.
+template <int PAD>
+struct LinePadAt {
+  float2* base;
+  __device__ __forceinline__ float2* operator()(int row) const { return base + row + row / PAD; }
+};
+
+// sin(x) / x as its Taylor series in q = x^2 (alternating terms (-q)^n / (2n+1)!, up to
+// n = 7): for the k-space correction x = (c_ref dt / 2) |k| <= (pi / 2) CFL sqrt(3) <= 1.64
+// (CFL <= 0.6), so q <= 2.7 and the first omitted term is below 1e-12.
+__device__ __forceinline__ float sinc_q(float q) {
+  float r = -1.f / 1307674368000.f;  // -1/15!
+  r = fmaf(r, q, 1.f / 6227020800.f);
+  r = fmaf(r, q, -1.f / 39916800.f);
+  r = fmaf(r, q, 1.f / 362880.f);
+  r = fmaf(r, q, -1.f / 5040.f);
+  r = fmaf(r, q, 1.f / 120.f);
+  r = fmaf(r, q, -1.f / 6.f);
+  return fmaf(r, q, 1.f);
 }
 
 // ================================================================ y pass
-// In-place C2C along y of one or three half spectra. Columns c = x * nzh + zb.
+// In-place C2C along y of one or three half spectra. Block = (x, column tile of TC bins).
 template <int L, int DIR>
 __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     k_ypass(StepArgs a, const float2* __restrict__ twg, int which) {
   using P = Plan<L>;
+  using S = Split<L, P::T>;
   constexpr int TC = P::TC;
+  constexpr int T = P::T;
   __shared__ float2 tile[L * TC];
   __shared__ float2 tw[L];
-  for (int i = threadIdx.x; i < L; i += TC * P::T) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < L; i += TC * T) tw[i] = twg[i];
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
   float2* __restrict__ data = which == 0 ? m.sp : m.sg[blockIdx.y];
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
-  const int nzh = a.s.nzh;
-  const int C = a.s.nx * nzh;
-  const int c = blockIdx.x * TC + col;
-  const bool valid = c < C;
-  const int cc = valid ? c : 0;
-  const long base = (long)(cc / nzh) * L * nzh + (cc % nzh);
-  const long S = nzh;
+  const int nt = (a.s.nzh + TC - 1) / TC;
+  const int x = blockIdx.x / nt;
+  const int zb = (blockIdx.x - x * nt) * TC + col;
+  const bool valid = zb < a.s.nzh;
+  const long S_ = a.s.ms;
+  const long base = (long)x * a.s.ps + (valid ? zb : 0);
   __syncthreads();
-  float2 V[P::R1][P::B];
-  in_load<L>(t, [&](int n) { return valid ? data[base + n * S] : make_float2(0.f, 0.f); }, V);
+  float2 V[S::R1][S::B];
+  in_load<L, T>(t, [&](int n) { return valid ? data[base + n * S_] : make_float2(0.f, 0.f); }, V);
   TileAt<TC> at{tile, col};
-  in_compute<L, DIR>(t, V, at, tw);
+  in_compute<L, T, DIR>(t, V, at, tw);
   __syncthreads();
-  float2 X[P::R2][P::A];
-  in_finish<L, DIR>(t, at, X);
+  float2 X[S::R2][S::A];
+  in_finish<L, T, DIR>(t, at, X);
   if (valid) {
 #pragma unroll
-    for (int r = 0; r < P::R2; ++r) {
-      const int k2 = t + P::T * r;
-      if (k2 < P::B) {
+    for (int r = 0; r < S::R2; ++r) {
+      const int k2 = t + T * r;
+      if (k2 < S::B) {
 #pragma unroll
-        for (int k1 = 0; k1 < P::A; ++k1) data[base + (long)(k2 + P::B * k1) * S] = X[r][k1];
+        for (int k1 = 0; k1 < S::A; ++k1) data[base + (long)(k2 + S::B * k1) * S_] = X[r][k1];
       }
     }
   }
 }
 
 // ================================================================ x passes
-// Columns c = y * nzh + zb are contiguous; element (kx, c) at kx * ny * nzh + c.
+// Block = (y, column tile of TC bins); element (kx, y, zb) at kx * ps + y * ms + zb.
 template <int L>
-__device__ __forceinline__ void x_kappa(const MemberDev& m, const Shape& s, int c, int t,
-                                        float (&kap)[Plan<L>::R2][Plan<L>::A], float& kyd,
-                                        float& kzd) {
-  using P = Plan<L>;
+__device__ __forceinline__ void x_kappa(const MemberDev& m, const Shape& s, int y, int zb, int t,
+                                        float (&kap)[Split<L, Plan<L>::T>::R2][Split<L, Plan<L>::T>::A],
+                                        float& kyd, float& kzd) {
+  using S = Split<L, Plan<L>::T>;
+  constexpr int T = Plan<L>::T;
   const float* kt = m.ktab;
   const float* kx_full = kt;
   const float* ky_full = kt + 2 * s.nx;
   const float* ky_d = ky_full + s.ny;
   const float* kz_full = ky_d + s.ny;
   const float* kz_d = kz_full + s.nzh;
-  const int y = c / s.nzh, zb = c - y * s.nzh;
   const float kyf = ky_full[y], kzf = kz_full[zb];
   const float kyz2 = kyf * kyf + kzf * kzf;
+  const float h2 = m.half_cdt * m.half_cdt;
   kyd = ky_d[y];
   kzd = kz_d[zb];
 #pragma unroll
-  for (int r = 0; r < P::R2; ++r) {
-    const int k2 = t + P::T * r;
+  for (int r = 0; r < S::R2; ++r) {
+    const int k2 = t + T * r;
 #pragma unroll
-    for (int k1 = 0; k1 < P::A; ++k1) {
-      const int kxi = (k2 < P::B) ? k2 + P::B * k1 : 0;
+    for (int k1 = 0; k1 < S::A; ++k1) {
+      const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
       const float kxf = kx_full[kxi];
-      kap[r][k1] = sincf_half(m.half_cdt * sqrtf(kxf * kxf + kyz2)) * m.inv_n;
+      kap[r][k1] = sinc_q(h2 * (kxf * kxf + kyz2)) * m.inv_n;
     }
   }
 }
@@ -228,55 +263,60 @@ template <int L>
 __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     k_xgrad(StepArgs a, const float2* __restrict__ twg) {
   using P = Plan<L>;
+  constexpr int T = P::T;
+  using S = Split<L, T>;
   constexpr int TC = P::TC;
   __shared__ float2 tile[L * TC];
   __shared__ float2 tw[L];
-  for (int i = threadIdx.x; i < L; i += TC * P::T) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < L; i += TC * T) tw[i] = twg[i];
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
-  const long C = (long)a.s.ny * a.s.nzh;
-  const long c = (long)blockIdx.x * TC + col;
-  const bool valid = c < C;
-  const long base = valid ? c : 0;
+  const int nt = (a.s.nzh + TC - 1) / TC;
+  const int y = blockIdx.x / nt;
+  const int zb0 = (blockIdx.x - y * nt) * TC + col;
+  const bool valid = zb0 < a.s.nzh;
+  const int zb = valid ? zb0 : 0;
+  const long C = a.s.ps;
+  const long base = (long)y * a.s.ms + zb;
   __syncthreads();
-  float2 V[P::R1][P::B];
+  float2 V[S::R1][S::B];
   const float2* __restrict__ src = m.sp;
-  in_load<L>(t, [&](int n) { return valid ? src[base + n * C] : make_float2(0.f, 0.f); }, V);
+  in_load<L, T>(t, [&](int n) { return valid ? src[base + n * C] : make_float2(0.f, 0.f); }, V);
   TileAt<TC> at{tile, col};
-  in_compute<L, -1>(t, V, at, tw);
+  in_compute<L, T, -1>(t, V, at, tw);
   __syncthreads();
-  float2 X[P::R2][P::A];
-  in_finish<L, -1>(t, at, X);
-  float kap[P::R2][P::A];
+  float2 X[S::R2][S::A];
+  in_finish<L, T, -1>(t, at, X);
+  float kap[S::R2][S::A];
   float kyd, kzd;
-  x_kappa<L>(m, a.s, (int)base, t, kap, kyd, kzd);
+  x_kappa<L>(m, a.s, y, zb, t, kap, kyd, kzd);
   const float* kx_d = m.ktab + a.s.nx;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
-    float2 Z[P::R2][P::A];
+    float2 Z[S::R2][S::A];
 #pragma unroll
-    for (int r = 0; r < P::R2; ++r) {
-      const int k2 = t + P::T * r;
+    for (int r = 0; r < S::R2; ++r) {
+      const int k2 = t + T * r;
 #pragma unroll
-      for (int k1 = 0; k1 < P::A; ++k1) {
-        const int kxi = (k2 < P::B) ? k2 + P::B * k1 : 0;
+      for (int k1 = 0; k1 < S::A; ++k1) {
+        const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
         const float kd = ax == 0 ? kx_d[kxi] : (ax == 1 ? kyd : kzd);
         const float d = kd * kap[r][k1];
         Z[r][k1] = make_float2(-d * X[r][k1].y, d * X[r][k1].x);
       }
     }
     __syncthreads();
-    out_compute<L, +1>(t, Z, at, tw);
+    out_compute<L, T, +1>(t, Z, at, tw);
     __syncthreads();
-    out_finish<L, +1>(t, at, V);
+    out_finish<L, T, +1>(t, at, V);
     if (valid) {
       float2* __restrict__ dst = m.sg[ax];
 #pragma unroll
-      for (int r = 0; r < P::R1; ++r) {
-        const int m2 = t + P::T * r;
-        if (m2 < P::A) {
+      for (int r = 0; r < S::R1; ++r) {
+        const int m2 = t + T * r;
+        if (m2 < S::A) {
 #pragma unroll
-          for (int j = 0; j < P::B; ++j) dst[base + (long)(m2 + P::A * j) * C] = V[r][j];
+          for (int j = 0; j < S::B; ++j) dst[base + (long)(m2 + S::A * j) * C] = V[r][j];
         }
       }
     }
@@ -288,38 +328,50 @@ template <int L>
 __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     k_xdiv(StepArgs a, const float2* __restrict__ twg) {
   using P = Plan<L>;
+  constexpr int T = P::T;
+  using S = Split<L, T>;
   constexpr int TC = P::TC;
   __shared__ float2 tile[L * TC];
   __shared__ float2 tw[L];
-  for (int i = threadIdx.x; i < L; i += TC * P::T) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < L; i += TC * T) tw[i] = twg[i];
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
-  const long C = (long)a.s.ny * a.s.nzh;
-  const long c = (long)blockIdx.x * TC + col;
-  const bool valid = c < C;
-  const long base = valid ? c : 0;
-  float kap[P::R2][P::A];
+  const int nt = (a.s.nzh + TC - 1) / TC;
+  const int y = blockIdx.x / nt;
+  const int zb0 = (blockIdx.x - y * nt) * TC + col;
+  const bool valid = zb0 < a.s.nzh;
+  const int zb = valid ? zb0 : 0;
+  const long C = a.s.ps;
+  const long base = (long)y * a.s.ms + zb;
+  float kap[S::R2][S::A];
   float kyd, kzd;
-  x_kappa<L>(m, a.s, (int)base, t, kap, kyd, kzd);
+  x_kappa<L>(m, a.s, y, zb, t, kap, kyd, kzd);
   const float* kx_d = m.ktab + a.s.nx;
   TileAt<TC> at{tile, col};
-  float2 Sacc[P::R2][P::A];
-  float2 V[P::R1][P::B];
+  float2 Sacc[S::R2][S::A];
+  float2 V[S::R1][S::B], Vn[S::R1][S::B];
+  in_load<L, T>(t, [&](int n) { return valid ? m.sg[0][base + n * C] : make_float2(0.f, 0.f); }, Vn);
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
-    const float2* __restrict__ src = m.sg[ax];
-    in_load<L>(t, [&](int n) { return valid ? src[base + n * C] : make_float2(0.f, 0.f); }, V);
-    __syncthreads();
-    in_compute<L, -1>(t, V, at, tw);
-    __syncthreads();
-    float2 X[P::R2][P::A];
-    in_finish<L, -1>(t, at, X);
 #pragma unroll
-    for (int r = 0; r < P::R2; ++r) {
-      const int k2 = t + P::T * r;
+    for (int r = 0; r < S::R1; ++r)
 #pragma unroll
-      for (int k1 = 0; k1 < P::A; ++k1) {
-        const int kxi = (k2 < P::B) ? k2 + P::B * k1 : 0;
+      for (int i = 0; i < S::B; ++i) V[r][i] = Vn[r][i];
+    if (ax < 2) {  // the next axis' column is in flight during this axis' transform
+      const float2* __restrict__ nsrc = m.sg[ax + 1];
+      in_load<L, T>(t, [&](int n) { return valid ? nsrc[base + n * C] : make_float2(0.f, 0.f); }, Vn);
+    }
+    __syncthreads();
+    in_compute<L, T, -1>(t, V, at, tw);
+    __syncthreads();
+    float2 X[S::R2][S::A];
+    in_finish<L, T, -1>(t, at, X);
+#pragma unroll
+    for (int r = 0; r < S::R2; ++r) {
+      const int k2 = t + T * r;
+#pragma unroll
+      for (int k1 = 0; k1 < S::A; ++k1) {
+        const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
         const float kd = ax == 0 ? kx_d[kxi] : (ax == 1 ? kyd : kzd);
         const float d = kd * kap[r][k1];
         const float2 term = make_float2(-d * X[r][k1].y, d * X[r][k1].x);
@@ -328,17 +380,17 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     }
   }
   __syncthreads();
-  out_compute<L, +1>(t, Sacc, at, tw);
+  out_compute<L, T, +1>(t, Sacc, at, tw);
   __syncthreads();
-  out_finish<L, +1>(t, at, V);
+  out_finish<L, T, +1>(t, at, V);
   if (valid) {
     float2* __restrict__ dst = m.sp;
 #pragma unroll
-    for (int r = 0; r < P::R1; ++r) {
-      const int m2 = t + P::T * r;
-      if (m2 < P::A) {
+    for (int r = 0; r < S::R1; ++r) {
+      const int m2 = t + T * r;
+      if (m2 < S::A) {
 #pragma unroll
-        for (int j = 0; j < P::B; ++j) dst[base + (long)(m2 + P::A * j) * C] = V[r][j];
+        for (int j = 0; j < S::B; ++j) dst[base + (long)(m2 + S::A * j) * C] = V[r][j];
       }
     }
   }
@@ -346,21 +398,10 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
 
 // ================================================================ z passes
 // Real length nz = 2M along the contiguous axis, computed as a length-M complex FFT of
-// z[m] = x[2m] + i x[2m+1] with the even/odd split in LDS (rt[k] = exp(2 pi i k / nz)).
-
-// Load M+1 half-spectrum bins of the block's lines into LDS (block-contiguous in HBM).
-template <int M>
-__device__ __forceinline__ void z_stage_spectrum(const float2* __restrict__ spec, long line0,
-                                                 int nlines_total, float2* lb) {
-  using Z = ZPlan<M>;
-  constexpr int NB = M + 1;
-  const long off = line0 * NB;
-  const int nl = cmin(Z::LPB, nlines_total - (int)line0);
-  for (int i = threadIdx.x; i < nl * NB; i += Z::THREADS) {
-    const int l = i / NB, k = i - l * NB;
-    lb[l * Z::LS + k] = spec[off + i];
-  }
-}
+// z[m] = x[2m] + i x[2m+1] with the even/odd split (rt[k] = exp(2 pi i k / nz)).
+// Per line, LDS holds a natural-order buffer (M + 1 bins, later the real line as M
+// float2) and an exchange buffer; all HBM traffic of the pass is contiguous: the
+// spectrum rows, and the real volumes as one float2 stream per block.
 
 // C2R pre-split: Z[k] = Fe[k] + i Fo[k], Fe = X[k] + conj(X[M-k]),
 // Fo = (X[k] - conj(X[M-k])) exp(+2 pi i k / nz); imaginary parts of X[0], X[M] ignored.
@@ -391,234 +432,366 @@ __device__ __forceinline__ float2 r2c_post(const float2* l, int k, const float2*
   return x;
 }
 
-// Forward R2C of X (thread-held, z[m] = x[2m] + i x[2m+1], m = k2 + B k1) to HBM.
-template <int M>
-__device__ __forceinline__ void z_r2c_store(int t, float2 (&X)[Plan<M>::R2][Plan<M>::A], float2* lb,
-                                            float2* lb_line, const float2* tw, const float2* rt,
-                                            float2* __restrict__ dst_line, bool valid) {
-  using P = Plan<M>;
-  LineAt at{lb_line};
-  __syncthreads();
-  out_compute<M, -1>(t, X, at, tw);
-  __syncthreads();
-  float2 V[P::R1][P::B];
-  out_finish<M, -1>(t, at, V);
-  __syncthreads();
+template <int M, int ZV>
+struct ZLds {
+  float2 a[ZPlan<M, ZV>::LPB * ZPlan<M, ZV>::LSA];
+  float2 e[ZPlan<M, ZV>::LPB * ZPlan<M, ZV>::LSE];
+  float2 tw[M];
+  float2 rt[M + 1];
+  float sxy[ZPlan<M, ZV>::LPB];
+  int lx[ZPlan<M, ZV>::LPB];
+  int ly[ZPlan<M, ZV>::LPB];
+  long row[ZPlan<M, ZV>::LPB];  // offset of each line's spectrum row
+  int sptr[ZPlan<M, ZV>::LPB + 1];  // source CSR bounds of the block's lines
+};
+
+// per-thread counts of the block's contiguous real chunk (float2) and spectrum bins
+template <int M, int ZV>
+struct ZCount {
+  static constexpr int EPT = (ZPlan<M, ZV>::LPB * M + ZPlan<M, ZV>::THREADS - 1) / ZPlan<M, ZV>::THREADS;
+  static constexpr int SPT = (ZPlan<M, ZV>::LPB * (M + 1) + ZPlan<M, ZV>::THREADS - 1) / ZPlan<M, ZV>::THREADS;
+};
+
+__device__ __forceinline__ long row_of(long line, const Shape& s) {
+  const long x = line / s.ny;
+  return x * s.ps + (line - x * s.ny) * s.ms;
+}
+
+template <int M, int ZV>
+__device__ __forceinline__ void z_setup(const float2* twg, const float2* rtg, const MemberDev& m,
+                                        const Shape& sh, long line0, int nl, ZLds<M, ZV>& s) {
+  for (int i = threadIdx.x; i < M; i += ZPlan<M, ZV>::THREADS) s.tw[i] = twg[i];
+  for (int i = threadIdx.x; i <= M; i += ZPlan<M, ZV>::THREADS) s.rt[i] = rtg[i];
+  if (threadIdx.x < nl) {
+    const int line = (int)line0 + threadIdx.x;
+    const int x = line / sh.ny, y = line - x * sh.ny;
+    const float* sx = m.sponge + 2 * M;
+    s.sxy[threadIdx.x] = sx[x] * sx[sh.nx + y];
+    s.lx[threadIdx.x] = x;
+    s.ly[threadIdx.x] = y;
+    s.row[threadIdx.x] = (long)x * sh.ps + (long)y * sh.ms;
+  }
+}
+
+// issue the loads of the block's spectrum rows (M + 1 bins per line) into registers
+template <int M, int ZV>
+__device__ __forceinline__ void z_spec_load(const float2* __restrict__ spec, long line0, int nl,
+                                            const Shape& sh, float2 (&r)[ZCount<M, ZV>::SPT]) {
+  constexpr int NB = M + 1;
+  const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;
 #pragma unroll
-  for (int r = 0; r < P::R1; ++r) {
-    const int m2 = t + P::T * r;
-    if (m2 < P::A) {
-#pragma unroll
-      for (int j = 0; j < P::B; ++j) lb_line[m2 + P::A * j] = V[r][j];
+  for (int j = 0; j < ZCount<M, ZV>::SPT; ++j) {
+    const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
+    if (i < nl * NB) {
+      const int l = i / NB, k = i - l * NB;
+      // lines of a block are consecutive; they cross at most a few x planes
+      int y = y0 + l, x = x0;
+      while (y >= sh.ny) {
+        y -= sh.ny;
+        ++x;
+      }
+      r[j] = spec[(long)x * sh.ps + (long)y * sh.ms + k];
     }
   }
-  __syncthreads();
-  if (valid) {
-    for (int k = t; k <= M; k += P::T) dst_line[k] = r2c_post<M>(lb_line, k, rt);
+}
+
+template <int M, int ZV>
+__device__ __forceinline__ void z_spec_put(int nl, const float2 (&r)[ZCount<M, ZV>::SPT], ZLds<M, ZV>& s) {
+  constexpr int NB = M + 1;
+#pragma unroll
+  for (int j = 0; j < ZCount<M, ZV>::SPT; ++j) {
+    const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
+    if (i < nl * NB) {
+      const int l = i / NB, k = i - l * NB;
+      s.a[l * ZPlan<M, ZV>::LSA + k] = r[j];
+    }
   }
 }
 
-// Inverse C2R of the spectrum line staged in LDS into thread-held z pairs X[r][k1].
-template <int M>
-__device__ __forceinline__ void z_c2r_regs(int t, float2* lb_line, const float2* tw, const float2* rt,
-                                           float2 (&X)[Plan<M>::R2][Plan<M>::A]) {
-  using P = Plan<M>;
-  LineAt at{lb_line};
-  float2 V[P::R1][P::B];
-  in_load<M>(t, [&](int n) { return c2r_pre<M>(lb_line, n, rt); }, V);
-  __syncthreads();
-  in_compute<M, +1>(t, V, at, tw);
-  __syncthreads();
-  in_finish<M, +1>(t, at, X);
+// issue the loads of a real volume's chunk (float2 stream) into registers
+template <int M, int ZV>
+__device__ __forceinline__ void z_vol_load(const float2* __restrict__ v, int nl,
+                                           float2 (&r)[ZCount<M, ZV>::EPT]) {
+#pragma unroll
+  for (int j = 0; j < ZCount<M, ZV>::EPT; ++j) {
+    const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
+    r[j] = (i < nl * M) ? v[i] : make_float2(0.f, 0.f);
+  }
 }
 
-template <int M>
-__device__ __forceinline__ void z_tables(const float2* twg, const float2* rtg, float2* tw, float2* rt) {
-  using Z = ZPlan<M>;
-  for (int i = threadIdx.x; i < M; i += Z::THREADS) tw[i] = twg[i];
-  for (int i = threadIdx.x; i <= M; i += Z::THREADS) rt[i] = rtg[i];
-}
-
-// velocity update for the three axes
-template <int M>
-__global__ void __launch_bounds__(ZPlan<M>::THREADS)
-    k_zu(StepArgs a, const float2* __restrict__ twg, const float2* __restrict__ rtg) {
-  using P = Plan<M>;
-  using Z = ZPlan<M>;
-  __shared__ float2 lb[Z::LPB * Z::LS];
-  __shared__ float2 tw[M];
-  __shared__ float2 rt[M + 1];
-  z_tables<M>(twg, rtg, tw, rt);
-  const MemberDev& m = a.mem[a.ids[blockIdx.z]];
-  const int nlines = a.s.nx * a.s.ny;
-  const int nz = 2 * M;
-  const long line0 = (long)blockIdx.x * Z::LPB;
-  const int ll = threadIdx.x / P::T, t = threadIdx.x % P::T;
-  const long line = line0 + ll;
-  const bool valid = line < nlines;
-  const long lsafe = valid ? line : 0;
-  float2* lbl = lb + ll * Z::LS;
-  const int x = (int)(lsafe / a.s.ny), y = (int)(lsafe - (long)x * a.s.ny);
-  const float* sp = m.sponge;
-  const float sxy = sp[nz + x] * sp[nz + a.s.nx + y];
-  const float* sz = sp;
-  const float2* __restrict__ dto = reinterpret_cast<const float2*>(m.dto_rho + lsafe * nz);
-  const float2* __restrict__ ab = reinterpret_cast<const float2*>(m.absorb + lsafe * nz);
-#pragma unroll 1
-  for (int ax = 0; ax < 3; ++ax) {
-    float2* __restrict__ spec = m.sg[ax];
-    __syncthreads();
-    z_stage_spectrum<M>(spec, line0, nlines, lb);
-    __syncthreads();
-    float2 X[P::R2][P::A];
-    z_c2r_regs<M>(t, lbl, tw, rt, X);
-    float2* __restrict__ u = reinterpret_cast<float2*>(m.u[ax] + lsafe * nz);
+// C2R of every staged line: natural buffer a -> real line (as M float2) back in a.
+template <int M, int ZV>
+__device__ __forceinline__ void z_c2r(int ll, int t, bool lvalid, ZLds<M, ZV>& s) {
+  using Z = ZPlan<M, ZV>;
+  using S = Split<M, Z::ZT>;
+  float2* la = s.a + ll * Z::LSA;
+  float2* le = s.e + ll * Z::LSE;
+  float2 V[S::R1][S::B];
+  in_load<M, Z::ZT>(t, [&](int n) { return c2r_pre<M>(la, n, s.rt); }, V);
+  __syncthreads();  // a consumed
+  if (lvalid) in_compute<M, Z::ZT, +1>(t, V, LinePadAt<S::A>{le}, s.tw);
+  __syncthreads();
+  float2 X[S::R2][S::A];
+  if (lvalid) {
+    in_finish<M, Z::ZT, +1>(t, LinePadAt<S::A>{le}, X);
 #pragma unroll
-    for (int r = 0; r < P::R2; ++r) {
-      const int k2 = t + P::T * r;
-      if (k2 < P::B) {
+    for (int r = 0; r < S::R2; ++r) {
+      const int k2 = t + Z::ZT * r;
+      if (k2 < S::B) {
 #pragma unroll
-        for (int k1 = 0; k1 < P::A; ++k1) {
-          const int mm = k2 + P::B * k1;
-          if (valid) {
-            const float2 g = X[r][k1];
-            float2 uv = u[mm];
-            const float2 d = dto[mm];
-            const float2 b = ab[mm];
-            const float2 s2 = reinterpret_cast<const float2*>(sz)[mm];
-            uv.x -= d.x * g.x;
-            uv.y -= d.y * g.y;
-            uv.x *= b.x;
-            uv.y *= b.y;
-            uv.x *= sxy * s2.x;
-            uv.y *= sxy * s2.y;
-            u[mm] = uv;
-            X[r][k1] = uv;
-          }
-        }
+        for (int k1 = 0; k1 < S::A; ++k1) la[k2 + S::B * k1] = X[r][k1];
       }
     }
-    z_r2c_store<M>(t, X, lb, lbl, tw, rt, spec + lsafe * (M + 1), valid);
+  }
+  __syncthreads();
+}
+
+// R2C of the real line in a (M float2) -> spectrum row in HBM.
+template <int M, int ZV>
+__device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s,
+                                      float2* __restrict__ dst_row) {
+  using Z = ZPlan<M, ZV>;
+  using S = Split<M, Z::ZT>;
+  float2* la = s.a + ll * Z::LSA;
+  float2* le = s.e + ll * Z::LSE;
+  float2 V[S::R1][S::B];
+  in_load<M, Z::ZT>(t, [&](int n) { return la[n]; }, V);
+  if (lvalid) in_compute<M, Z::ZT, -1>(t, V, LinePadAt<S::A>{le}, s.tw);
+  __syncthreads();  // e written, a consumed
+  if (lvalid) {
+    float2 X[S::R2][S::A];
+    in_finish<M, Z::ZT, -1>(t, LinePadAt<S::A>{le}, X);
+#pragma unroll
+    for (int r = 0; r < S::R2; ++r) {
+      const int k2 = t + Z::ZT * r;
+      if (k2 < S::B) {
+#pragma unroll
+        for (int k1 = 0; k1 < S::A; ++k1) la[k2 + S::B * k1] = X[r][k1];
+      }
+    }
+  }
+  __syncthreads();
+  if (lvalid) {
+    for (int k = t; k <= M; k += Z::ZT) dst_row[k] = r2c_post<M>(la, k, s.rt);
   }
 }
 
-// pressure update (+ Westervelt), source, convergence peak, record; or (mode 0) a plain
-// R2C of the current p.
-template <int M>
-__global__ void __launch_bounds__(ZPlan<M>::THREADS)
+// velocity update, one axis per block. Block b covers line group g and axis ax with
+// b = 24 * (g / 8) + 8 * ax + (g % 8): blocks are dealt to the 8 XCDs round-robin, so
+// the three axis blocks of a line group run back to back on one XCD and its L2 serves
+// the second and third reads of dt/rho and absorb.
+template <int M, int ZV>
+__global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
+    k_zu(StepArgs a, const float2* __restrict__ twg, const float2* __restrict__ rtg) {
+  using Z = ZPlan<M, ZV>;
+  using CN = ZCount<M, ZV>;
+  __shared__ ZLds<M, ZV> s;
+  const MemberDev& m = a.mem[a.ids[blockIdx.z]];
+  const int nlines = a.s.nx * a.s.ny;
+  const int ngroups = (nlines + Z::LPB - 1) / Z::LPB;
+  const int b = blockIdx.x;
+  const int g = (b / 24) * 8 + (b % 8);
+  const int ax = (b / 8) % 3;
+  if (g >= ngroups) return;
+  const long line0 = (long)g * Z::LPB;
+  const int nl = cmin(Z::LPB, nlines - (int)line0);
+  const int ll = threadIdx.x / Z::ZT, t = threadIdx.x % Z::ZT;
+  const bool lvalid = ll < nl;
+  const long e0 = line0 * M;  // first float2 of the block's chunk
+  const float2* __restrict__ sz = reinterpret_cast<const float2*>(m.sponge);
+  float2* __restrict__ spec = m.sg[ax];
+  float2* __restrict__ u = reinterpret_cast<float2*>(m.u[ax]) + e0;
+  float2 sr[CN::SPT], uu[CN::EPT], dto[CN::EPT], ab[CN::EPT];
+  z_spec_load<M, ZV>(spec, line0, nl, a.s, sr);
+  z_vol_load<M, ZV>(u, nl, uu);
+  z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.dto_rho) + e0, nl, dto);
+  z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.absorb) + e0, nl, ab);
+  z_setup<M, ZV>(twg, rtg, m, a.s, line0, nl, s);
+  z_spec_put<M, ZV>(nl, sr, s);
+  __syncthreads();
+  z_c2r<M, ZV>(ll, t, lvalid, s);
+#pragma unroll
+  for (int j = 0; j < CN::EPT; ++j) {
+    const int i = threadIdx.x + j * Z::THREADS;
+    if (i < nl * M) {
+      const int l = i / M, mm = i - l * M;
+      const float sxy = s.sxy[l];
+      const float2 gg = s.a[l * Z::LSA + mm];
+      const float2 s2 = sz[mm];
+      float2 uv = uu[j];
+      uv.x -= dto[j].x * gg.x;
+      uv.y -= dto[j].y * gg.y;
+      uv.x *= ab[j].x;
+      uv.y *= ab[j].y;
+      uv.x *= sxy * s2.x;
+      uv.y *= sxy * s2.y;
+      u[i] = uv;
+      s.a[l * Z::LSA + mm] = uv;
+    }
+  }
+  __syncthreads();
+  z_r2c<M, ZV>(ll, t, lvalid, s, spec + s.row[lvalid ? ll : 0]);
+}
+
+// pressure update (+ Westervelt), source, convergence peak, record; or (update == 0) a
+// plain R2C of the current p. After the C2R every element is finished in one pass:
+// update, the source voxels of its line (values computed during the prefetch and
+// parked in LDS), the store of p, the convergence peak and the record sums.
+template <int M, int ZV>
+__global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     k_zp(StepArgs a, const float2* __restrict__ twg, const float2* __restrict__ rtg, int update) {
-  using P = Plan<M>;
-  using Z = ZPlan<M>;
-  __shared__ float2 lb[Z::LPB * Z::LS];
-  __shared__ float2 tw[M];
-  __shared__ float2 rt[M + 1];
-  z_tables<M>(twg, rtg, tw, rt);
+  using Z = ZPlan<M, ZV>;
+  using CN = ZCount<M, ZV>;
+  constexpr int kSrcPT = 2;                       // prefetched source entries per thread
+  constexpr int kSrcCap = kSrcPT * Z::THREADS;    // entries a block keeps in LDS
+  __shared__ ZLds<M, ZV> s;
+  __shared__ int src_zl[kSrcCap];
+  __shared__ float src_val[kSrcCap];
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
   const StepMember& sm = a.sm[blockIdx.z];
   const int nlines = a.s.nx * a.s.ny;
-  const int nz = 2 * M;
   const long line0 = (long)blockIdx.x * Z::LPB;
-  const int ll = threadIdx.x / P::T, t = threadIdx.x % P::T;
-  const long line = line0 + ll;
-  const bool valid = line < nlines;
-  const long lsafe = valid ? line : 0;
-  float2* lbl = lb + ll * Z::LS;
-  const int x = (int)(lsafe / a.s.ny), y = (int)(lsafe - (long)x * a.s.ny);
-  float2* __restrict__ pl = reinterpret_cast<float2*>(m.p + lsafe * nz);
-  float2 X[P::R2][P::A];
+  const int nl = cmin(Z::LPB, nlines - (int)line0);
+  const int ll = threadIdx.x / Z::ZT, t = threadIdx.x % Z::ZT;
+  const bool lvalid = ll < nl;
+  const long e0 = line0 * M;
+  float2* __restrict__ p = reinterpret_cast<float2*>(m.p) + e0;
+  float2 pp[CN::EPT];
   if (update) {
-    __syncthreads();
-    z_stage_spectrum<M>(m.sp, line0, nlines, lb);
-    __syncthreads();
-    z_c2r_regs<M>(t, lbl, tw, rt, X);
-    const float* sp = m.sponge;
-    const float sxy = sp[nz + x] * sp[nz + a.s.nx + y];
-    const float2* __restrict__ sz = reinterpret_cast<const float2*>(sp);
-    const float2* __restrict__ rc = reinterpret_cast<const float2*>(m.rhoc2dt + lsafe * nz);
-    const float2* __restrict__ ab = reinterpret_cast<const float2*>(m.absorb + lsafe * nz);
-    const float2* __restrict__ bt =
-        m.beta2dt ? reinterpret_cast<const float2*>(m.beta2dt + lsafe * nz) : nullptr;
+    const int sbeg = m.src_ptr[line0], send = m.src_ptr[line0 + nl];
+    const bool src_fit = send - sbeg <= kSrcCap;  // block-uniform
+    int sz_pf[kSrcPT];
+    float2 sw_pf[kSrcPT];
 #pragma unroll
-    for (int r = 0; r < P::R2; ++r) {
-      const int k2 = t + P::T * r;
-      if (k2 < P::B) {
-#pragma unroll
-        for (int k1 = 0; k1 < P::A; ++k1) {
-          const int mm = k2 + P::B * k1;
-          if (valid) {
-            const float2 dv = X[r][k1];
-            float2 pv = pl[mm];
-            float2 c = rc[mm];
-            if (bt) {
-              const float2 b = bt[mm];
-              c.x += b.x * pv.x;
-              c.y += b.y * pv.y;
-            }
-            const float2 b = ab[mm];
-            const float2 s2 = sz[mm];
-            pv.x -= c.x * dv.x;
-            pv.y -= c.y * dv.y;
-            pv.x *= b.x;
-            pv.y *= b.y;
-            pv.x *= sxy * s2.x;
-            pv.y *= sxy * s2.y;
-            X[r][k1] = pv;
-          }
-        }
+    for (int q = 0; q < kSrcPT; ++q) {
+      const int j = sbeg + threadIdx.x + q * Z::THREADS;
+      if (j < send) {
+        sz_pf[q] = m.src_z[j];
+        sw_pf[q] = m.src_ab[j];
       }
     }
-    // source voxels on this line
-    if (valid) {
-      const int beg = m.src_ptr[lsafe], end = m.src_ptr[lsafe + 1];
-      for (int j = beg; j < end; ++j) {
-        const int zz = m.src_z[j];
-        const int mm = zz >> 1;
-        const int k2 = mm % P::B, k1s = mm / P::B;
-        const float2 w = m.src_ab[j];
-        const float val = sm.amp * (w.x * sm.sin_t - w.y * sm.cos_t);
+    if (threadIdx.x <= nl) s.sptr[threadIdx.x] = m.src_ptr[line0 + threadIdx.x];
+    float2 sr[CN::SPT], rc[CN::EPT], ab[CN::EPT], bt[CN::EPT];
+    z_spec_load<M, ZV>(m.sp, line0, nl, a.s, sr);
+    z_vol_load<M, ZV>(p, nl, pp);
+    z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.rhoc2dt) + e0, nl, rc);
+    z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.absorb) + e0, nl, ab);
+    if (m.beta2dt) z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.beta2dt) + e0, nl, bt);
+    z_setup<M, ZV>(twg, rtg, m, a.s, line0, nl, s);
 #pragma unroll
-        for (int r = 0; r < P::R2; ++r) {
-#pragma unroll
-          for (int k1 = 0; k1 < P::A; ++k1) {
-            if (k2 == t + P::T * r && k1 == k1s) {
-              if (zz & 1) X[r][k1].y += val;
-              else X[r][k1].x += val;
-            }
-          }
-        }
+    for (int q = 0; q < kSrcPT; ++q) {
+      const int j = sbeg + threadIdx.x + q * Z::THREADS;
+      if (j < send) {
+        src_zl[j - sbeg] = sz_pf[q];
+        src_val[j - sbeg] = sm.amp * (sw_pf[q].x * sm.sin_t - sw_pf[q].y * sm.cos_t);
       }
     }
-    // store p, convergence peak, record
-    unsigned pk = 0u;
-    const bool in_conv_line = valid && x >= m.conv[0] && x < m.conv[1] && y >= m.conv[2] && y < m.conv[3];
-    const bool in_rec_line = valid && sm.record && x >= m.rec[0] && x < m.rec[1] && y >= m.rec[2] &&
-                             y < m.rec[3];
+    z_spec_put<M, ZV>(nl, sr, s);
+    __syncthreads();
+    z_c2r<M, ZV>(ll, t, lvalid, s);
+    const float2* __restrict__ sz = reinterpret_cast<const float2*>(m.sponge);
     const int ry = m.rec[3] - m.rec[2], rz = m.rec[5] - m.rec[4];
-    const long rline = ((long)(x - m.rec[0]) * ry + (y - m.rec[2])) * rz - m.rec[4];
+    unsigned pk = 0u;
 #pragma unroll
-    for (int r = 0; r < P::R2; ++r) {
-      const int k2 = t + P::T * r;
-      if (k2 < P::B) {
-#pragma unroll
-        for (int k1 = 0; k1 < P::A; ++k1) {
-          const int mm = k2 + P::B * k1;
-          if (valid) {
-            const float2 pv = X[r][k1];
-            pl[mm] = pv;
+    for (int j = 0; j < CN::EPT; ++j) {
+      const int i = threadIdx.x + j * Z::THREADS;
+      if (i < nl * M) {
+        const int l = i / M, mm = i - l * M;
+        const float sxy = s.sxy[l];
+        const float2 dv = s.a[l * Z::LSA + mm];
+        float2 pv = pp[j];
+        float2 c = rc[j];
+        if (m.beta2dt) {
+          c.x += bt[j].x * pv.x;
+          c.y += bt[j].y * pv.y;
+        }
+        const float2 s2 = sz[mm];
+        pv.x -= c.x * dv.x;
+        pv.y -= c.y * dv.y;
+        pv.x *= ab[j].x;
+        pv.y *= ab[j].y;
+        pv.x *= sxy * s2.x;
+        pv.y *= sxy * s2.y;
+        if (src_fit) {
+          for (int e = s.sptr[l]; e < s.sptr[l + 1]; ++e) {
+            const int zz = src_zl[e - sbeg];
+            if ((zz >> 1) == mm) {
+              if (zz & 1) pv.y += src_val[e - sbeg];
+              else pv.x += src_val[e - sbeg];
+            }
+          }
+        }
+        s.a[l * Z::LSA + mm] = pv;
+        if (src_fit) {
+          const int x = s.lx[l], y = s.ly[l];
+          p[i] = pv;
+          if (x >= m.conv[0] && x < m.conv[1] && y >= m.conv[2] && y < m.conv[3]) {
+            const int z0 = 2 * mm;
+            if (z0 >= m.conv[4] && z0 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.x)));
+            if (z0 + 1 >= m.conv[4] && z0 + 1 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.y)));
+          }
+          if (sm.record && x >= m.rec[0] && x < m.rec[1] && y >= m.rec[2] && y < m.rec[3]) {
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
               const int zz = 2 * mm + e;
               const float v = e ? pv.y : pv.x;
-              if (in_conv_line && zz >= m.conv[4] && zz < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(v)));
-              if (in_rec_line && zz >= m.rec[4] && zz < m.rec[5]) {
-                const long ri = rline + zz;
+              if (zz >= m.rec[4] && zz < m.rec[5]) {
+                const long ri = ((long)(x - m.rec[0]) * ry + (y - m.rec[2])) * rz + (zz - m.rec[4]);
                 for (int h = 0; h < m.nh; ++h) {
                   const float2 eh = sm.eh[h];
-                  float2 s = m.acc[h][ri], cp = m.comp[h][ri];
+                  float2 sacc = m.acc[h][ri], cp = m.comp[h][ri];
                   const float yx = v * eh.x - cp.x, yy = v * eh.y - cp.y;
-                  const float tx = s.x + yx, ty = s.y + yy;
-                  cp.x = (tx - s.x) - yx;
-                  cp.y = (ty - s.y) - yy;
+                  const float tx = sacc.x + yx, ty = sacc.y + yy;
+                  cp.x = (tx - sacc.x) - yx;
+                  cp.y = (ty - sacc.y) - yy;
+                  m.acc[h][ri] = make_float2(tx, ty);
+                  m.comp[h][ri] = cp;
+                }
+                m.pmax[ri] = fmaxf(m.pmax[ri], fabsf(v));
+              }
+            }
+          }
+        }
+      }
+    }
+    if (!src_fit) {
+      // more source voxels than the LDS list holds: add them in a separate pass
+      __syncthreads();
+      for (int j = sbeg + threadIdx.x; j < send; j += Z::THREADS) {
+        int l = 0;
+        while (l + 1 < nl && s.sptr[l + 1] <= j) ++l;
+        const float2 w = m.src_ab[j];
+        float* lf = reinterpret_cast<float*>(s.a + l * Z::LSA);
+        lf[m.src_z[j]] += sm.amp * (w.x * sm.sin_t - w.y * sm.cos_t);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < CN::EPT; ++j) {
+        const int i = threadIdx.x + j * Z::THREADS;
+        if (i < nl * M) {
+          const int l = i / M, mm = i - l * M;
+          const int x = s.lx[l], y = s.ly[l];
+          const float2 pv = s.a[l * Z::LSA + mm];
+          p[i] = pv;
+          if (x >= m.conv[0] && x < m.conv[1] && y >= m.conv[2] && y < m.conv[3]) {
+            const int z0 = 2 * mm;
+            if (z0 >= m.conv[4] && z0 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.x)));
+            if (z0 + 1 >= m.conv[4] && z0 + 1 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.y)));
+          }
+          if (sm.record && x >= m.rec[0] && x < m.rec[1] && y >= m.rec[2] && y < m.rec[3]) {
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+              const int zz = 2 * mm + e;
+              const float v = e ? pv.y : pv.x;
+              if (zz >= m.rec[4] && zz < m.rec[5]) {
+                const long ri = ((long)(x - m.rec[0]) * ry + (y - m.rec[2])) * rz + (zz - m.rec[4]);
+                for (int h = 0; h < m.nh; ++h) {
+                  const float2 eh = sm.eh[h];
+                  float2 sacc = m.acc[h][ri], cp = m.comp[h][ri];
+                  const float yx = v * eh.x - cp.x, yy = v * eh.y - cp.y;
+                  const float tx = sacc.x + yx, ty = sacc.y + yy;
+                  cp.x = (tx - sacc.x) - yx;
+                  cp.y = (ty - sacc.y) - yy;
                   m.acc[h][ri] = make_float2(tx, ty);
                   m.comp[h][ri] = cp;
                 }
@@ -630,21 +803,27 @@ __global__ void __launch_bounds__(ZPlan<M>::THREADS)
       }
     }
     if (sm.peak_idx >= 0) {
+      // one atomic per wave, spread over the period's slots by wave id
       for (int off = 32; off > 0; off >>= 1) pk = max(pk, (unsigned)__shfl_xor((int)pk, off));
-      if ((threadIdx.x & 63) == 0 && pk) atomicMax(m.peaks + sm.peak_idx, pk);
+      if ((threadIdx.x & 63) == 0 && pk) {
+        const int slot = (blockIdx.x * ((Z::THREADS + 63) / 64) + (threadIdx.x >> 6)) % kPeakSlots;
+        atomicMax(m.peaks + (long)sm.peak_idx * kPeakWords + slot * kPeakStride, pk);
+      }
     }
   } else {
+    z_vol_load<M, ZV>(p, nl, pp);
+    z_setup<M, ZV>(twg, rtg, m, a.s, line0, nl, s);
 #pragma unroll
-    for (int r = 0; r < P::R2; ++r) {
-      const int k2 = t + P::T * r;
-#pragma unroll
-      for (int k1 = 0; k1 < P::A; ++k1) {
-        const int mm = (k2 < P::B) ? k2 + P::B * k1 : 0;
-        X[r][k1] = valid ? pl[mm] : make_float2(0.f, 0.f);
+    for (int j = 0; j < CN::EPT; ++j) {
+      const int i = threadIdx.x + j * Z::THREADS;
+      if (i < nl * M) {
+        const int l = i / M, mm = i - l * M;
+        s.a[l * Z::LSA + mm] = pp[j];
       }
     }
   }
-  z_r2c_store<M>(t, X, lb, lbl, tw, rt, m.sp + lsafe * (M + 1), valid);
+  __syncthreads();
+  z_r2c<M, ZV>(ll, t, lvalid, s, m.sp + s.row[lvalid ? ll : 0]);
 }
 
 }  // namespace cst

@@ -18,8 +18,10 @@ bool length_supported(int L);
 bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, hipStream_t st);
 bool launch_xgrad(int L, const StepArgs& a, const float2* tw, hipStream_t st);
 bool launch_xdiv(int L, const StepArgs& a, const float2* tw, hipStream_t st);
-bool launch_zu(int M, const StepArgs& a, const float2* tw, const float2* rt, hipStream_t st);
-bool launch_zp(int M, const StepArgs& a, const float2* tw, const float2* rt, int update,
+// variant: thread layout of the z passes (ZPlan<M, V>), chosen per session
+bool launch_zu(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt,
+               hipStream_t st);
+bool launch_zp(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt, int update,
                hipStream_t st);
 
 }  // namespace cst

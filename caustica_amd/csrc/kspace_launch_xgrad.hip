@@ -5,12 +5,11 @@
 namespace cst {
 
 bool launch_xgrad(int L, const StepArgs& a, const float2* tw, hipStream_t st) {
-  const long C = (long)a.s.ny * a.s.nzh;
   switch (L) {
 #define CST_XG(N)                                                                \
   case N: {                                                                      \
     using P = Plan<N>;                                                           \
-    dim3 g((unsigned)((C + P::TC - 1) / P::TC), 1, a.nmem);                      \
+    dim3 g((unsigned)(a.s.ny * ((a.s.nzh + P::TC - 1) / P::TC)), 1, a.nmem);     \
     hipLaunchKernelGGL((k_xgrad<N>), g, dim3(P::TC * P::T), 0, st, a, tw);       \
     return true;                                                                 \
   }

@@ -4,16 +4,23 @@
 
 namespace cst {
 
-bool launch_zu(int M, const StepArgs& a, const float2* tw, const float2* rt, hipStream_t st) {
+template <int N, int V>
+static void go_zu(const StepArgs& a, const float2* tw, const float2* rt, hipStream_t st) {
+  using Z = ZPlan<N, V>;
   const int nl = a.s.nx * a.s.ny;
+  const int ng = (nl + Z::LPB - 1) / Z::LPB;
+  dim3 g(24 * ((ng + 7) / 8), 1, a.nmem);
+  hipLaunchKernelGGL((k_zu<N, V>), g, dim3(Z::THREADS), 0, st, a, tw, rt);
+}
+
+bool launch_zu(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt,
+               hipStream_t st) {
   switch (M) {
-#define CST_ZU(N)                                                                      \
-  case N: {                                                                            \
-    using Z = ZPlan<N>;                                                                \
-    dim3 g((nl + Z::LPB - 1) / Z::LPB, 1, a.nmem);                                     \
-    hipLaunchKernelGGL((k_zu<N>), g, dim3(Z::THREADS), 0, st, a, tw, rt);              \
-    return true;                                                                       \
-  }
+#define CST_ZU(N)                                  \
+  case N:                                          \
+    if (variant == 1) go_zu<N, 1>(a, tw, rt, st);  \
+    else go_zu<N, 0>(a, tw, rt, st);               \
+    return true;
     CST_LENGTHS(CST_ZU)
 #undef CST_ZU
     default:

@@ -2,7 +2,8 @@
 //
 // Layout in HBM for one solve ("member") on a padded grid nx x ny x nz (z contiguous):
 //   real volumes   [nx][ny][nz]      float     p, u0, u1, u2, dt/rho, rho c^2 dt, absorb, 2 beta dt
-//   half spectra   [nx][ny][nz/2+1]  float2    sp (p-hat, then the divergence spectrum),
+//   half spectra   [nx][ps]: [ny][ms] float2 (ms = nz/2+1 rounded up to 16 bins,
+//                                              ps = ny ms + 16)    sp (p-hat, then the divergence spectrum),
 //                                              sg0..sg2 (gradient spectra, then u-hat)
 //   record box     [rx][ry][rz]      float2    Kahan sum + compensation per harmonic; float p_max
 // Several members of one shape share every launch (blockIdx.z indexes the member list).
@@ -13,6 +14,11 @@ namespace cst {
 
 constexpr int kMaxBatch = 16;     // members per launch
 constexpr int kMaxHarmonics = 3;  // harmonics recorded per member
+// Per-period convergence peaks are spread over kPeakSlots words, kPeakStride uints
+// apart (64 bytes), so that blocks do not all hit one address with their atomicMax.
+constexpr int kPeakSlots = 64;
+constexpr int kPeakStride = 16;
+constexpr int kPeakWords = kPeakSlots * kPeakStride;  // per period
 
 struct MemberDev {
   float* p;
@@ -33,7 +39,7 @@ struct MemberDev {
   float2* acc[kMaxHarmonics];
   float2* comp[kMaxHarmonics];
   float* pmax;
-  unsigned* peaks;       // per-period max |p| over the convergence box (float bits)
+  unsigned* peaks;       // [period][kPeakWords]: max |p| over the convergence box (float bits)
   int conv[6];           // x0, x1, y0, y1, z0, z1 (half-open)
   int rec[6];
   int nh;
@@ -54,6 +60,8 @@ struct StepMember {
 
 struct Shape {
   int nx, ny, nz, nzh;
+  int ms;   // spectrum row stride in bins: nzh rounded up to a multiple of 16
+  long ps;  // spectrum x-plane stride in bins: ny * ms + 16 (breaks power-of-two strides)
 };
 
 struct StepArgs {

@@ -30,6 +30,7 @@ struct Member {
   bool finalized[kMaxHarmonics] = {false, false, false};
   bool primed = true;
   void* block = nullptr;
+  size_t block_bytes = 0;
   MemberDev dev{};
 };
 
@@ -57,6 +58,8 @@ struct cst_session {
   float2* tw_y = nullptr;
   float2* tw_z = nullptr;
   float2* rt = nullptr;
+  std::vector<std::pair<size_t, void*>> pool;  // retired member blocks kept for reuse
+  int variant[4] = {0, 0, 0, 0};               // per tunable pass: 0 y, 1 x, 2 zu, 3 zp
   Profiler prof;
 };
 
@@ -105,11 +108,11 @@ int launch_step(cst_session* s, StepArgs& a) {
   bool ok = true;
   ok = ok && s->prof.run(0, st, [&] { return launch_xgrad(sh.nx, a, s->tw_x, st); });
   if (sh.ny > 1) ok = ok && s->prof.run(1, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 1, st); });
-  ok = ok && s->prof.run(2, st, [&] { return launch_zu(M, a, s->tw_z, s->rt, st); });
+  ok = ok && s->prof.run(2, st, [&] { return launch_zu(M, s->variant[2], a, s->tw_z, s->rt, st); });
   if (sh.ny > 1) ok = ok && s->prof.run(1, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 1, st); });
   ok = ok && s->prof.run(3, st, [&] { return launch_xdiv(sh.nx, a, s->tw_x, st); });
   if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 0, st); });
-  ok = ok && s->prof.run(5, st, [&] { return launch_zp(M, a, s->tw_z, s->rt, 1, st); });
+  ok = ok && s->prof.run(5, st, [&] { return launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 1, st); });
   if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 0, st); });
   if (!ok) return fail(CST_ERR_UNSUPPORTED, "no kernel instantiated for this shape");
   HIPCHK(hipGetLastError());
@@ -125,7 +128,7 @@ int prime(cst_session* s, int slot) {
   a.ids[0] = slot;
   a.sm[0].peak_idx = -1;
   const int M = s->s.nz / 2;
-  if (!launch_zp(M, a, s->tw_z, s->rt, 0, s->stream))
+  if (!launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 0, s->stream))
     return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
   if (s->s.ny > 1 && !launch_y(s->s.ny, -1, a, s->tw_y, 0, s->stream))
     return fail(CST_ERR_UNSUPPORTED, "no kernel for ny");
@@ -166,7 +169,10 @@ int cst_session_create(int nx, int ny, int nz, int max_members, int device, cst_
   HIPCHK(hipSetDevice(device));
   auto* s = new cst_session();
   s->device = device;
-  s->s = Shape{nx, ny, nz, nz / 2 + 1};
+  {
+    const int ms = ((nz / 2 + 1 + 15) / 16) * 16;
+    s->s = Shape{nx, ny, nz, nz / 2 + 1, ms, (long)ny * ms + 16};
+  }
   s->max_members = max_members;
   s->members.resize(max_members);
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -205,6 +211,7 @@ int cst_session_destroy(cst_session* s) {
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (auto& m : s->members)
     if (m.block) (void)hipFree(m.block);
+  for (auto& b : s->pool) (void)hipFree(b.second);
   if (s->d_table) (void)hipFree(s->d_table);
   if (s->d_tables) (void)hipFree(s->d_tables);
   s->prof.release();
@@ -275,7 +282,7 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
 
   // one device allocation, carved in 256-byte aligned pieces
   const int64_t P = (int64_t)sh.nx * sh.ny * sh.nz;
-  const int64_t H = (int64_t)sh.nx * sh.ny * sh.nzh;
+  const int64_t H = (int64_t)sh.nx * sh.ps;
   const int64_t R = (int64_t)(d->rec[1] - d->rec[0]) * (d->rec[3] - d->rec[2]) * (d->rec[5] - d->rec[4]);
   const int nmaps = d->beta2_dt ? 4 : 3;
   const int64_t nk = 2LL * (sh.nx + sh.ny + sh.nzh);
@@ -292,13 +299,20 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   const size_t o_sab = c.take(src_ab.size() * 8);
   const size_t o_rec = c.take(2 * d->n_harmonics * R * 8);
   const size_t o_pmax = c.take(R * 4);
-  const size_t o_peaks = c.take(maxp * 4);
+  const size_t o_peaks = c.take((int64_t)maxp * kPeakWords * 4);
   HIPCHK(hipSetDevice(s->device));
   Member& m = s->members[slot];
   char* base = nullptr;
-  HIPCHK(hipMalloc((void**)&base, c.total));
+  for (size_t i = 0; i < s->pool.size(); ++i)
+    if (s->pool[i].first == c.total) {
+      base = (char*)s->pool[i].second;
+      s->pool.erase(s->pool.begin() + i);
+      break;
+    }
+  if (!base) HIPCHK(hipMalloc((void**)&base, c.total));
   m = Member{};
   m.block = base;
+  m.block_bytes = c.total;
   m.used = true;
   MemberDev& md = m.dev;
   float* st = (float*)(base + o_state);
@@ -358,11 +372,11 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   if (e == hipSuccess) e = up((void*)md.src_ab, src_ab.data(), src_ab.size() * 8);
   if (e == hipSuccess) e = hipMemsetAsync(base + o_rec, 0, 2 * d->n_harmonics * R * 8, stm);
   if (e == hipSuccess) e = hipMemsetAsync(base + o_pmax, 0, R * 4, stm);
-  if (e == hipSuccess) e = hipMemsetAsync(md.peaks, 0, maxp * 4, stm);
+  if (e == hipSuccess) e = hipMemsetAsync(md.peaks, 0, (size_t)maxp * kPeakWords * 4, stm);
   if (e == hipSuccess) e = up(s->d_table + slot, &md, sizeof(MemberDev));
   if (e == hipSuccess) e = hipStreamSynchronize(stm);
   if (e != hipSuccess) {
-    (void)hipFree(base);
+    s->pool.emplace_back(c.total, base);
     m = Member{};
     return fail(CST_ERR_HIP, std::string("member upload: ") + hipGetErrorString(e));
   }
@@ -375,8 +389,38 @@ int cst_member_remove(cst_session* s, int slot) {
   if (int r = check_slot(s, slot)) return r;
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipStreamSynchronize(s->stream));
-  HIPCHK(hipFree(s->members[slot].block));
+  // keep the block for the next member of this shape (hipFree would stall every stream)
+  s->pool.emplace_back(s->members[slot].block_bytes, s->members[slot].block);
   s->members[slot] = Member{};
+  return CST_OK;
+}
+
+int cst_session_trim(cst_session* s, size_t* released) {
+  if (!s) return fail(CST_ERR_ARG, "null session");
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  size_t n = 0;
+  for (auto& b : s->pool) {
+    n += b.first;
+    HIPCHK(hipFree(b.second));
+  }
+  s->pool.clear();
+  if (released) *released = n;
+  return CST_OK;
+}
+
+int cst_session_pooled_bytes(cst_session* s, size_t* bytes) {
+  if (!s || !bytes) return fail(CST_ERR_ARG, "null argument");
+  size_t n = 0;
+  for (auto& b : s->pool) n += b.first;
+  *bytes = n;
+  return CST_OK;
+}
+
+int cst_device_memory(int device, size_t* free_bytes, size_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return fail(CST_ERR_ARG, "null output");
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipMemGetInfo(free_bytes, total_bytes));
   return CST_OK;
 }
 
@@ -385,7 +429,7 @@ int cst_member_reset(cst_session* s, int slot) {
   Member& m = s->members[slot];
   const Shape& sh = s->s;
   const int64_t P = (int64_t)sh.nx * sh.ny * sh.nz;
-  const int64_t H = (int64_t)sh.nx * sh.ny * sh.nzh;
+  const int64_t H = (int64_t)sh.nx * sh.ps;
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipMemsetAsync(m.dev.p, 0, 4 * P * 4, s->stream));
   HIPCHK(hipMemsetAsync(m.dev.sp, 0, 4 * H * 8, s->stream));
@@ -395,7 +439,7 @@ int cst_member_reset(cst_session* s, int slot) {
     m.finalized[h] = false;
   }
   HIPCHK(hipMemsetAsync(m.dev.pmax, 0, m.rec_count * 4, s->stream));
-  HIPCHK(hipMemsetAsync(m.dev.peaks, 0, (size_t)m.max_periods * 4, s->stream));
+  HIPCHK(hipMemsetAsync(m.dev.peaks, 0, (size_t)m.max_periods * kPeakWords * 4, s->stream));
   m.n = 0;
   m.record_from = INT64_MAX;
   m.primed = true;
@@ -460,8 +504,15 @@ int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out)
     return fail(CST_ERR_ARG, "peak range outside the member's period capacity");
   if (count == 0) return CST_OK;
   HIPCHK(hipSetDevice(s->device));
-  HIPCHK(hipMemcpyAsync(out, m.dev.peaks + first, count * 4, hipMemcpyDeviceToHost, s->stream));
+  std::vector<unsigned> raw((size_t)count * kPeakWords);
+  HIPCHK(hipMemcpyAsync(raw.data(), m.dev.peaks + (size_t)first * kPeakWords,
+                        raw.size() * 4, hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < count; ++i) {
+    unsigned b = 0u;
+    for (int k = 0; k < kPeakSlots; ++k) b = std::max(b, raw[(size_t)i * kPeakWords + k * kPeakStride]);
+    std::memcpy(out + i, &b, 4);
+  }
   return CST_OK;
 }
 
@@ -518,6 +569,14 @@ int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, co
   HIPCHK(hipStreamSynchronize(s->stream));
   m.n = n;
   m.primed = false;
+  return CST_OK;
+}
+
+int cst_session_set_variant(cst_session* s, int pass, int variant) {
+  if (!s) return fail(CST_ERR_ARG, "null session");
+  if (pass < 0 || pass > 3 || variant < 0 || variant > 1)
+    return fail(CST_ERR_ARG, "pass must be 0..3 and variant 0..1");
+  s->variant[pass] = variant;
   return CST_OK;
 }
 

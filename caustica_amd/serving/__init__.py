@@ -1,0 +1,11 @@
+"""Continuous batching of CW solves (mixed grid sizes, HBM admission, batched launches)."""
+
+from caustica_amd.serving.batcher import (
+    ContinuousBatcher,
+    SolveRequest,
+    SolveResponse,
+    serve_serial,
+    summarize,
+)
+
+__all__ = ["ContinuousBatcher", "SolveRequest", "SolveResponse", "serve_serial", "summarize"]

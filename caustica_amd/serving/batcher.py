@@ -1,0 +1,309 @@
+"""Continuous batching of CW solves of mixed grid sizes on one GPU.
+
+Requests arrive over time. Each is planned on the host (coefficient maps, k tables,
+source; a small thread pool so planning overlaps device work), then admitted into the
+session of its padded shape when a member slot is free and its HBM footprint fits the
+budget (footprint = MemberPlan.device_bytes(), the device-side inventory of a member;
+upstream's planner estimate for the same solve, planner/model.py:86-115, is larger
+because it counts dense operator volumes the device never builds). Every session
+advances all of its members together -- one batched launch sequence per step, each
+member at its own step counter, so members join and leave at any step boundary --
+and different shapes run on their own streams, so their kernels overlap on the
+device. A member whose schedule is done is read back and retired; its slot and memory
+are reused by the next admission of that shape.
+
+A member's trajectory does not depend on what it shares launches with: every kernel
+reads only the member's own arrays and scalars, so a result is bit-identical alone
+and in the mix (tests/test_gpu_serving.py).
+"""
+
+from __future__ import annotations
+
+import queue
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from caustica_amd.solvers.base import SolverDivergedError, SolverResult
+from caustica_amd.solvers.kspace.native import KSpaceSession
+from caustica_amd.solvers.kspace.operators import MemberPlan, plan_member
+
+
+@dataclass
+class SolveRequest:
+    """One CW solve with a pinned schedule (convergence_tol == 0: its step count is known)."""
+
+    rid: int
+    arrival_s: float
+    solver: str
+    grid: object
+    medium: object
+    source: object
+    spec: object
+    harmonics: tuple = (1,)
+    reference_point: tuple | None = None
+    record_region: tuple | None = None
+    label: str = ""
+
+
+@dataclass
+class SolveResponse:
+    rid: int
+    result: SolverResult | None
+    error: str | None
+    arrival_s: float
+    admit_s: float
+    done_s: float
+    shape: tuple
+    batch_peak: int = 1
+
+    @property
+    def latency_s(self) -> float:
+        return self.done_s - self.arrival_s
+
+
+@dataclass
+class _Live:
+    req: SolveRequest
+    plan: MemberPlan
+    slot: int
+    total_steps: int
+    rec_steps: int
+    admit_s: float
+    done_steps: int = 0
+    peak_batch: int = 1
+
+
+@dataclass
+class _Group:
+    session: KSpaceSession
+    live: dict = field(default_factory=dict)  # slot -> _Live
+
+
+def schedule_of(plan: MemberPlan, spec) -> tuple[int, int]:
+    """(total steps, record steps) of a pinned schedule (engine.py:484-556 with tol = 0)."""
+    if spec.convergence_tol != 0.0:
+        raise ValueError("served solves need a pinned schedule (convergence_tol == 0)")
+    settle = plan.eff_max
+    if spec.t_end_min_us is not None:
+        settle = max(settle, int(np.ceil(spec.t_end_min_us * 1e-6 * plan.f0)) - spec.n_record_periods)
+    rec = spec.n_record_periods * plan.spp
+    return settle * plan.spp + rec, rec
+
+
+def finish_result(sess: KSpaceSession, slot: int, plan: MemberPlan, req: SolveRequest,
+                  total_steps: int, rec_steps: int, periods: int) -> SolverResult:
+    """Read a finished member back into the upstream result shape (engine.py:558-586)."""
+    peaks = sess.peaks(slot, 0, plan.eff_max)
+    history, prev = [], None
+    for p in range(1, plan.eff_max + 1):
+        pk = float(peaks[p - 1])
+        if not np.isfinite(pk):
+            raise SolverDivergedError(f"request {req.rid} diverged in settling period {p}")
+        rel = abs(pk - prev) / max(prev, 1e-9) if prev is not None else None
+        if p >= plan.tof_periods:
+            history.append((p, pk, float("nan") if rel is None else rel))
+        prev = pk
+    phasors, pmax = {}, None
+    for hi, h in enumerate(plan.harmonics):
+        ph, pm = sess.result(slot, hi, 2.0 / rec_steps, with_pmax=(hi == 0))
+        phasors[h] = ph.reshape(plan.rec_shape)
+        if hi == 0:
+            pmax = pm.reshape(plan.rec_shape)
+    if not np.isfinite(pmax).all():
+        raise SolverDivergedError(f"request {req.rid} diverged in the record window")
+    return SolverResult(
+        phasor=phasors[1], p_max=pmax, region=plan.rec, dt=plan.dt, spp=plan.spp,
+        steps_total=total_steps, t_end_s=total_steps * plan.dt, tof_periods=plan.tof_periods,
+        converged_period=plan.eff_max, settle_capped=True, convergence_history=history,
+        phasors=phasors,
+        meta={"solver": req.solver, "backend": "hip", "padded_shape": plan.padded,
+              "c_ref": plan.c_ref, "nonlinear_active": plan.beta2_dt is not None,
+              "source": plan.label, "served": True},
+    )
+
+
+class ContinuousBatcher:
+    """Serve a stream of CW solve requests with per-shape batched launches.
+
+    hbm_budget: bytes of device memory members may hold at once (default: 80 % of
+    the free device memory at construction). chunk_steps: steps every live member
+    advances between host checks (admission / retirement granularity).
+    """
+
+    def __init__(self, hbm_budget: int | None = None, max_batch: int = 16, chunk_steps: int = 16,
+                 planners: int = 4, device: int = 0):
+        from caustica_amd import _native as N
+
+        self.device = device
+        if hbm_budget is None:
+            free, _total = N.device_memory(device)
+            hbm_budget = int(0.8 * free)
+        self.hbm_budget = int(hbm_budget)
+        self.max_batch = int(max_batch)
+        self.chunk_steps = int(chunk_steps)
+        self.groups: dict[tuple, _Group] = {}
+        self.in_use = 0
+        self.planners = ThreadPoolExecutor(max_workers=planners)
+
+    def _group(self, padded3) -> _Group:
+        g = self.groups.get(padded3)
+        if g is None:
+            g = _Group(KSpaceSession(padded3, self.max_batch, self.device))
+            self.groups[padded3] = g
+        return g
+
+    def tune(self, plan: MemberPlan) -> None:
+        """Autotune the pass variants of a shape group before serving (cached per shape)."""
+        from caustica_amd.tune import autotune
+
+        autotune(self._group(plan.padded3).session, plan)
+
+    @staticmethod
+    def _plan(req: SolveRequest) -> MemberPlan:
+        return plan_member(req.grid, req.medium, req.source, req.spec, harmonics=req.harmonics,
+                           record_region=req.record_region, reference_point=req.reference_point,
+                           nonlinear=req.solver == "westervelt")
+
+    def close(self) -> None:
+        self.planners.shutdown(wait=True)
+        for g in self.groups.values():
+            g.session.close()
+        self.groups.clear()
+
+    def serve(self, requests: list[SolveRequest], t0: float | None = None,
+              on_done=None) -> list[SolveResponse]:
+        """Replay ``requests`` by their arrival times (seconds after t0) and serve them.
+
+        Returns one response per request (in completion order)."""
+        pending = sorted(requests, key=lambda r: r.arrival_s)
+        t0 = time.perf_counter() if t0 is None else t0
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        planned: queue.Queue = queue.Queue()
+        waiting: list = []  # (req, plan) planned but not admitted
+        responses: list[SolveResponse] = []
+        n_total = len(pending)
+        nxt = 0
+        in_flight_plans = 0
+
+        def submit_plan(req):
+            fut = self.planners.submit(self._plan, req)
+            fut.add_done_callback(lambda f, r=req: planned.put((r, f)))
+
+        while len(responses) < n_total:
+            # 1. arrivals -> planner pool
+            t = now()
+            while nxt < n_total and pending[nxt].arrival_s <= t:
+                submit_plan(pending[nxt])
+                in_flight_plans += 1
+                nxt += 1
+            # 2. planned -> waiting
+            while True:
+                try:
+                    req, fut = planned.get_nowait()
+                except queue.Empty:
+                    break
+                in_flight_plans -= 1
+                try:
+                    waiting.append((req, fut.result()))
+                except Exception as exc:  # noqa: BLE001 - a bad request fails alone
+                    responses.append(SolveResponse(req.rid, None, f"{type(exc).__name__}: {exc}",
+                                                   req.arrival_s, now(), now(), tuple(req.grid.shape)))
+            # 3. admission by arrival order, HBM budget and free slots
+            still = []
+            for req, plan in waiting:
+                g = self._group(plan.padded3)
+                need = plan.device_bytes()
+                pooled = sum(x.session.pooled_bytes() for x in self.groups.values())
+                if self.in_use + need + pooled > self.hbm_budget and pooled:
+                    for x in self.groups.values():
+                        x.session.trim()  # retired blocks of other shapes give way
+                    pooled = 0
+                if len(g.live) < self.max_batch and self.in_use + need + pooled <= self.hbm_budget:
+                    total, rec = schedule_of(plan, req.spec)
+                    slot = g.session.add(plan, max_periods=plan.eff_max + 4)
+                    g.session.schedule(slot, total - rec)
+                    g.live[slot] = _Live(req, plan, slot, total, rec, now())
+                    self.in_use += need
+                else:
+                    still.append((req, plan))
+            waiting = still
+            # 4. one chunk for every group with live members (async, overlapping streams)
+            active = [g for g in self.groups.values() if g.live]
+            if not active:
+                if nxt < n_total and not waiting and in_flight_plans == 0:
+                    time.sleep(max(0.0, min(pending[nxt].arrival_s - now(), 0.002)))
+                else:
+                    time.sleep(0.0005)
+                continue
+            for g in active:
+                k = min(self.chunk_steps, min(lv.total_steps - lv.done_steps for lv in g.live.values()))
+                g.session.advance(list(g.live), k)
+                for lv in g.live.values():
+                    lv.done_steps += k
+                    lv.peak_batch = max(lv.peak_batch, len(g.live))
+            for g in active:
+                g.session.sync()
+            # 5. retire finished members
+            for g in active:
+                for slot in [s for s, lv in g.live.items() if lv.done_steps >= lv.total_steps]:
+                    lv = g.live.pop(slot)
+                    err, res = None, None
+                    try:
+                        res = finish_result(g.session, slot, lv.plan, lv.req, lv.total_steps,
+                                            lv.rec_steps, 0)
+                    except Exception as exc:  # noqa: BLE001
+                        err = f"{type(exc).__name__}: {exc}"
+                    g.session.remove(slot)
+                    self.in_use -= lv.plan.device_bytes()
+                    resp = SolveResponse(lv.req.rid, res, err, lv.req.arrival_s, lv.admit_s, now(),
+                                         tuple(lv.req.grid.shape), lv.peak_batch)
+                    responses.append(resp)
+                    if on_done is not None:
+                        on_done(resp)
+        return responses
+
+
+def serve_serial(requests: list[SolveRequest], t0: float | None = None) -> list[SolveResponse]:
+    """The plain form: one solve at a time through the solver API, in arrival order."""
+    import caustica_amd.solvers as solvers
+
+    pending = sorted(requests, key=lambda r: r.arrival_s)
+    t0 = time.perf_counter() if t0 is None else t0
+    out = []
+    for req in pending:
+        wait = req.arrival_s - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        start = time.perf_counter() - t0
+        err, res = None, None
+        try:
+            res = solvers.get(req.solver)().run(req.grid, req.medium, req.source, req.spec,
+                                                backend="hip", harmonics=req.harmonics,
+                                                reference_point=req.reference_point,
+                                                record_region=req.record_region)
+        except Exception as exc:  # noqa: BLE001
+            err = f"{type(exc).__name__}: {exc}"
+        out.append(SolveResponse(req.rid, res, err, req.arrival_s, start,
+                                 time.perf_counter() - t0, tuple(req.grid.shape)))
+    return out
+
+
+def summarize(responses: list[SolveResponse]) -> dict:
+    ok = [r for r in responses if r.error is None]
+    lat = np.array([r.latency_s for r in ok]) if ok else np.zeros(1)
+    first = min(r.arrival_s for r in responses)
+    last = max(r.done_s for r in responses)
+    return {
+        "requests": len(responses),
+        "errors": len(responses) - len(ok),
+        "solves_per_s": len(ok) / max(last - first, 1e-9),
+        "makespan_s": last - first,
+        "latency_p50_s": float(np.percentile(lat, 50)),
+        "latency_p99_s": float(np.percentile(lat, 99)),
+        "latency_max_s": float(lat.max()),
+        "max_batch_seen": max((r.batch_peak for r in responses), default=1),
+    }

@@ -40,10 +40,13 @@ from caustica_amd.solvers.base import (
 )
 from caustica_amd.solvers.kspace.native import KSpaceSession
 from caustica_amd.solvers.kspace.operators import MemberPlan, plan_member
+from caustica_amd.tune import autotune
 
 log = logging.getLogger("caustica")
 
 SCHEME = "cw-kspace-pstd/2"
+#: grids at least this large get their pass variants autotuned (cached per shape)
+TUNE_MIN_VOXELS = 1 << 21
 
 
 def guard_finite(peak: float, *, where: str, solver_name: str) -> None:
@@ -120,7 +123,10 @@ def run_cw_kspace_pstd(
     if spec.t_end_min_us is not None:
         extra = max(0, ceil(spec.t_end_min_us * 1e-6 / period) - spec.n_record_periods - plan.eff_max)
     max_periods = plan.eff_max + extra + spec.n_record_periods + 4
-    with KSpaceSession(plan.padded3, 1, b.device) as sess:
+    nvox = plan.padded3[0] * plan.padded3[1] * plan.padded3[2]
+    with KSpaceSession(plan.padded3, 2, b.device) as sess:
+        if nvox >= TUNE_MIN_VOXELS:
+            autotune(sess, plan)
         slot = sess.add(plan, max_periods=max_periods)
         return _control_loop(sess, slot, plan, solver_name, grid, medium, source, spec, b,
                              reference_point, checkpoint, progress)

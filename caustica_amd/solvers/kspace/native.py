@@ -107,6 +107,16 @@ class KSpaceSession:
         N.check(self._lib.cst_member_remove(self._h, int(slot)))
         self._plans.pop(slot, None)
 
+    def trim(self) -> int:
+        n = C.c_size_t(0)
+        N.check(self._lib.cst_session_trim(self._h, C.byref(n)))
+        return int(n.value)
+
+    def pooled_bytes(self) -> int:
+        n = C.c_size_t(0)
+        N.check(self._lib.cst_session_pooled_bytes(self._h, C.byref(n)))
+        return int(n.value)
+
     def reset(self, slot: int) -> None:
         N.check(self._lib.cst_member_reset(self._h, int(slot)))
 
@@ -156,6 +166,9 @@ class KSpaceSession:
         arrs = [np.ascontiguousarray(np.reshape(a, self.padded3), np.float32) for a in (p, u0, u1, u2)]
         N.check(self._lib.cst_member_set_state(self._h, int(slot), C.c_int64(int(n)),
                                                *[N.ptr(a) for a in arrs]))
+
+    def set_variant(self, pass_: int, variant: int) -> None:
+        N.check(self._lib.cst_session_set_variant(self._h, int(pass_), int(variant)))
 
     # ---------------------------------------------------------------- timing
     def set_profiling(self, on: bool) -> None:

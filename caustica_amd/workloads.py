@@ -53,20 +53,25 @@ class Workload:
 
 
 def gaussian_random_field(shape, dx: float, corr_len: float, seed: int) -> np.ndarray:
-    """Unit-variance GRF with Gaussian correlation of standard deviation corr_len."""
+    """Unit-variance GRF with Gaussian correlation of standard deviation corr_len (float32)."""
+    import os
+
+    import scipy.fft as sfft
+
     rng = np.random.default_rng(seed)
     noise = rng.standard_normal(shape, dtype=np.float32)
     sigma_vox = corr_len / np.sqrt(2.0) / dx
-    out = noise
+    workers = min(16, os.cpu_count() or 1)
+    spec = sfft.rfftn(noise, workers=workers)
     for ax, n in enumerate(shape):
-        k = np.fft.rfftfreq(n)  # cycles per voxel
+        k = np.fft.rfftfreq(n) if ax == len(shape) - 1 else np.fft.fftfreq(n)  # cycles / voxel
         filt = np.exp(-2.0 * (np.pi * k * sigma_vox) ** 2).astype(np.float32)
         sh = [1] * len(shape)
         sh[ax] = filt.size
-        out = np.fft.irfft(np.fft.rfft(out, axis=ax) * filt.reshape(sh), n=n, axis=ax)
-    out = out.astype(np.float64)
-    out -= out.mean()
-    out /= out.std()
+        spec *= filt.reshape(sh)
+    out = sfft.irfftn(spec, s=shape, workers=workers)
+    out -= out.mean(dtype=np.float64)
+    out /= out.std(dtype=np.float64)
     return out.astype(np.float32)
 
 
@@ -153,3 +158,56 @@ def cfg2_westervelt(seed: int = 2, shape=(256, 256, 384)) -> Workload:
                        reference_point=focus)
     return Workload(f"cfg2_westervelt_{'x'.join(map(str, shape))}", "westervelt", grid, medium,
                     src, spec, (1, 2, 3), 80, reference_point=focus)
+
+
+SERVING_SHAPES = ((128, 128, 128), (192, 192, 192), (256, 256, 256), (128, 128, 256))
+
+
+def serving_request(rid: int, rng: np.random.Generator, arrival_s: float, shape=None,
+                    solver: str | None = None):
+    """One request of the mixed-size queue: config-1 medium family at dx 0.375 mm,
+    fresh random-field seed, 70 % linear / 30 % Westervelt (harmonics 1, 2), bowl focus
+    steered by a random offset (+-10 mm lateral, +-15 mm axial) with delay-and-sum
+    phases. Geometry per size: ROC = min(64 mm, 0.6 Lz), aperture radius =
+    min(32 mm, 0.8 ROC, 0.4 L_lateral); the steered focus is clipped to stay at least
+    PML + 4 voxels inside the grid. Pinned schedule: time of flight to the focus + 10
+    settle + 2 record periods, CFL 0.3."""
+    from caustica_amd.serving.batcher import SolveRequest
+
+    dx = 0.375e-3
+    if shape is None:
+        shape = SERVING_SHAPES[int(rng.integers(len(SERVING_SHAPES)))]
+    if solver is None:
+        solver = "westervelt" if rng.random() < 0.3 else "linear"
+    grid = Grid(shape, dx, PMLSpec(thickness=16 * dx))
+    medium = layered_medium(shape, dx, seed=int(rng.integers(1 << 30)), nonlinear=solver == "westervelt")
+    lz = shape[2] * dx
+    lat = min(shape[0], shape[1]) * dx
+    roc = min(64e-3, 0.6 * lz)
+    a = min(32e-3, 0.8 * roc, 0.4 * lat)
+    apex = (shape[0] // 2, shape[1] // 2, grid.pml_vox)
+    off = np.array([rng.uniform(-10e-3, 10e-3), rng.uniform(-10e-3, 10e-3), rng.uniform(-15e-3, 15e-3)])
+    focus = np.array(apex, np.float64) + np.array([0.0, 0.0, roc / dx]) + off / dx
+    lo = grid.pml_vox + 4
+    focus = tuple(int(round(np.clip(f, lo, n - 1 - lo))) for f, n in zip(focus, shape))
+    src = bowl_cw_source(grid, F0, 2.0e5, aperture_radius=a, roc=roc, apex_vox=apex,
+                         phases=lambda idx: focus_phases(idx, dx, focus, 1500.0, F0))
+    tof = cw_tof_periods(shape, dx, medium.c_min, src.indices, F0, focus)
+    spec = pinned_spec(grid, medium, src, cfl=0.3, periods_total=tof + 12, n_record=2,
+                       reference_point=focus)
+    harmonics = (1, 2) if solver == "westervelt" else (1,)
+    return SolveRequest(rid=rid, arrival_s=arrival_s, solver=solver, grid=grid, medium=medium,
+                        source=src, spec=spec, harmonics=harmonics, reference_point=focus,
+                        label=f"{solver} {'x'.join(map(str, shape))}")
+
+
+def serving_queue(n: int, rate_hz: float, seed: int = 4, shapes=None):
+    """n requests with Poisson arrivals at rate_hz (seeded exponential inter-arrival times)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate_hz)
+        shape = None if shapes is None else shapes[int(rng.integers(len(shapes)))]
+        out.append(serving_request(rid, rng, t, shape=shape))
+    return out

@@ -93,7 +93,14 @@ typedef struct {
 /* Upload a member; returns its slot. Allocates its device memory (state, spectra,
  * maps, record box). */
 int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot);
+/* Retire a member. Its device block stays with the session and is reused by the next
+ * member of the same footprint (no hipFree on the serving path). */
 int cst_member_remove(cst_session* s, int slot);
+/* Free the session's retired blocks; *released (may be NULL) = bytes returned. */
+int cst_session_trim(cst_session* s, size_t* released);
+int cst_session_pooled_bytes(cst_session* s, size_t* bytes);
+/* Free and total device memory (hipMemGetInfo) of a device, for admission budgets. */
+int cst_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
 /* Zero the member's state, spectra, record sums and peaks and set its step counter to
  * 0, keeping its uploaded maps and source (a fresh solve of the same setup). */
 int cst_member_reset(cst_session* s, int slot);
@@ -122,6 +129,10 @@ int cst_member_get_state(cst_session* s, int slot, float* p, float* u0, float* u
                          int64_t* n);
 int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, const float* u0,
                          const float* u1, const float* u2);
+
+/* Select the thread-layout variant (0 or 1) of a tunable pass (0 y, 1 x, 2 zu, 3 zp) for
+ * every later step of the session; the autotuner times both and keeps the faster. */
+int cst_session_set_variant(cst_session* s, int pass, int variant);
 
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the session
  * stream and its duration accumulated per kernel kind. Kinds: 0 xgrad, 1 y (3 spectra),
