@@ -37,6 +37,7 @@ EXPORTS = (
     "cst_session_trim",
     "cst_session_pooled_bytes",
     "cst_device_memory",
+    "cst_diag_copy",
     "cst_member_schedule",
     "cst_member_step",
     "cst_advance",
@@ -137,6 +138,7 @@ def _bind(lib):
         "cst_session_trim": ([v, C.POINTER(C.c_size_t)], i),
         "cst_session_pooled_bytes": ([v, C.POINTER(C.c_size_t)], i),
         "cst_device_memory": ([i, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], i),
+        "cst_diag_copy": ([i, C.c_size_t], i),
         "cst_member_schedule": ([v, i, i64], i),
         "cst_member_step": ([v, i, C.POINTER(i64)], i),
         "cst_advance": ([v, i, C.POINTER(i), i], i),

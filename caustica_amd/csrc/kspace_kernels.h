@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   __shared__ float2 tw[L];
   for (int i = threadIdx.x; i < L; i += TC * T) tw[i] = twg[i];
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
-  float2* __restrict__ data = which == 0 ? m.sp : m.sg[blockIdx.y];
+  const gptr<float2> data = G(which == 0 ? m.sp : m.sg[blockIdx.y]);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
   const int nt = (a.s.nzh + TC - 1) / TC;
   const int x = blockIdx.x / nt;
@@ -209,7 +209,7 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const long base = (long)x * a.s.ps + (valid ? zb : 0);
   __syncthreads();
   float2 V[S::R1][S::B];
-  in_load<L, T>(t, [&](int n) { return valid ? data[base + n * S_] : make_float2(0.f, 0.f); }, V);
+  in_load<L, T>(t, [&](int n) { return valid ? ld(data, base + n * S_) : make_float2(0.f, 0.f); }, V);
   TileAt<TC> at{tile, col};
   in_compute<L, T, DIR>(t, V, at, tw);
   __syncthreads();
@@ -235,12 +235,12 @@ __device__ __forceinline__ void x_kappa(const MemberDev& m, const Shape& s, int 
                                         float& kyd, float& kzd) {
   using S = Split<L, Plan<L>::T>;
   constexpr int T = Plan<L>::T;
-  const float* kt = m.ktab;
-  const float* kx_full = kt;
-  const float* ky_full = kt + 2 * s.nx;
-  const float* ky_d = ky_full + s.ny;
-  const float* kz_full = ky_d + s.ny;
-  const float* kz_d = kz_full + s.nzh;
+  const gptr<const float> kt = G(m.ktab);
+  const gptr<const float> kx_full = kt;
+  const gptr<const float> ky_full = kt + 2 * s.nx;
+  const gptr<const float> ky_d = ky_full + s.ny;
+  const gptr<const float> kz_full = ky_d + s.ny;
+  const gptr<const float> kz_d = kz_full + s.nzh;
   const float kyf = ky_full[y], kzf = kz_full[zb];
   const float kyz2 = kyf * kyf + kzf * kzf;
   const float h2 = m.half_cdt * m.half_cdt;
@@ -280,8 +280,8 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const long base = (long)y * a.s.ms + zb;
   __syncthreads();
   float2 V[S::R1][S::B];
-  const float2* __restrict__ src = m.sp;
-  in_load<L, T>(t, [&](int n) { return valid ? src[base + n * C] : make_float2(0.f, 0.f); }, V);
+  const gptr<const float2> src = G(m.sp);
+  in_load<L, T>(t, [&](int n) { return valid ? ld(src, base + n * C) : make_float2(0.f, 0.f); }, V);
   TileAt<TC> at{tile, col};
   in_compute<L, T, -1>(t, V, at, tw);
   __syncthreads();
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   float kap[S::R2][S::A];
   float kyd, kzd;
   x_kappa<L>(m, a.s, y, zb, t, kap, kyd, kzd);
-  const float* kx_d = m.ktab + a.s.nx;
+  const gptr<const float> kx_d = G(m.ktab) + a.s.nx;
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
     float2 Z[S::R2][S::A];
@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     __syncthreads();
     out_finish<L, T, +1>(t, at, V);
     if (valid) {
-      float2* __restrict__ dst = m.sg[ax];
+      const gptr<float2> dst = G(m.sg[ax]);
 #pragma unroll
       for (int r = 0; r < S::R1; ++r) {
         const int m2 = t + T * r;
@@ -346,11 +346,12 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   float kap[S::R2][S::A];
   float kyd, kzd;
   x_kappa<L>(m, a.s, y, zb, t, kap, kyd, kzd);
-  const float* kx_d = m.ktab + a.s.nx;
+  const gptr<const float> kx_d = G(m.ktab) + a.s.nx;
   TileAt<TC> at{tile, col};
   float2 Sacc[S::R2][S::A];
   float2 V[S::R1][S::B], Vn[S::R1][S::B];
-  in_load<L, T>(t, [&](int n) { return valid ? m.sg[0][base + n * C] : make_float2(0.f, 0.f); }, Vn);
+  const gptr<const float2> src0 = G(m.sg[0]);
+  in_load<L, T>(t, [&](int n) { return valid ? ld(src0, base + n * C) : make_float2(0.f, 0.f); }, Vn);
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
 #pragma unroll
@@ -358,8 +359,8 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
 #pragma unroll
       for (int i = 0; i < S::B; ++i) V[r][i] = Vn[r][i];
     if (ax < 2) {  // the next axis' column is in flight during this axis' transform
-      const float2* __restrict__ nsrc = m.sg[ax + 1];
-      in_load<L, T>(t, [&](int n) { return valid ? nsrc[base + n * C] : make_float2(0.f, 0.f); }, Vn);
+      const gptr<const float2> nsrc = G(m.sg[ax + 1]);
+      in_load<L, T>(t, [&](int n) { return valid ? ld(nsrc, base + n * C) : make_float2(0.f, 0.f); }, Vn);
     }
     __syncthreads();
     in_compute<L, T, -1>(t, V, at, tw);
@@ -384,7 +385,7 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   __syncthreads();
   out_finish<L, T, +1>(t, at, V);
   if (valid) {
-    float2* __restrict__ dst = m.sp;
+    const gptr<float2> dst = G(m.sp);
 #pragma unroll
     for (int r = 0; r < S::R1; ++r) {
       const int m2 = t + T * r;
@@ -465,7 +466,7 @@ __device__ __forceinline__ void z_setup(const float2* twg, const float2* rtg, co
   if (threadIdx.x < nl) {
     const int line = (int)line0 + threadIdx.x;
     const int x = line / sh.ny, y = line - x * sh.ny;
-    const float* sx = m.sponge + 2 * M;
+    const gptr<const float> sx = G(m.sponge) + 2 * M;
     s.sxy[threadIdx.x] = sx[x] * sx[sh.nx + y];
     s.lx[threadIdx.x] = x;
     s.ly[threadIdx.x] = y;
@@ -475,7 +476,7 @@ __device__ __forceinline__ void z_setup(const float2* twg, const float2* rtg, co
 
 // issue the loads of the block's spectrum rows (M + 1 bins per line) into registers
 template <int M, int ZV>
-__device__ __forceinline__ void z_spec_load(const float2* __restrict__ spec, long line0, int nl,
+__device__ __forceinline__ void z_spec_load(gptr<const float2> spec, long line0, int nl,
                                             const Shape& sh, float2 (&r)[ZCount<M, ZV>::SPT]) {
   constexpr int NB = M + 1;
   const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;
@@ -510,12 +511,12 @@ __device__ __forceinline__ void z_spec_put(int nl, const float2 (&r)[ZCount<M, Z
 
 // issue the loads of a real volume's chunk (float2 stream) into registers
 template <int M, int ZV>
-__device__ __forceinline__ void z_vol_load(const float2* __restrict__ v, int nl,
+__device__ __forceinline__ void z_vol_load(gptr<const float2> v, int nl,
                                            float2 (&r)[ZCount<M, ZV>::EPT]) {
 #pragma unroll
   for (int j = 0; j < ZCount<M, ZV>::EPT; ++j) {
     const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
-    r[j] = (i < nl * M) ? v[i] : make_float2(0.f, 0.f);
+    r[j] = (i < nl * M) ? ld(v, i) : make_float2(0.f, 0.f);
   }
 }
 
@@ -549,7 +550,7 @@ __device__ __forceinline__ void z_c2r(int ll, int t, bool lvalid, ZLds<M, ZV>& s
 // R2C of the real line in a (M float2) -> spectrum row in HBM.
 template <int M, int ZV>
 __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s,
-                                      float2* __restrict__ dst_row) {
+                                      gptr<float2> dst_row) {
   using Z = ZPlan<M, ZV>;
   using S = Split<M, Z::ZT>;
   float2* la = s.a + ll * Z::LSA;
@@ -598,14 +599,14 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   const int ll = threadIdx.x / Z::ZT, t = threadIdx.x % Z::ZT;
   const bool lvalid = ll < nl;
   const long e0 = line0 * M;  // first float2 of the block's chunk
-  const float2* __restrict__ sz = reinterpret_cast<const float2*>(m.sponge);
-  float2* __restrict__ spec = m.sg[ax];
-  float2* __restrict__ u = reinterpret_cast<float2*>(m.u[ax]) + e0;
+  const gptr<const float2> sz = G(reinterpret_cast<const float2*>(m.sponge));
+  const gptr<float2> spec = G(m.sg[ax]);
+  const gptr<float2> u = G(reinterpret_cast<float2*>(m.u[ax])) + e0;
   float2 sr[CN::SPT], uu[CN::EPT], dto[CN::EPT], ab[CN::EPT];
   z_spec_load<M, ZV>(spec, line0, nl, a.s, sr);
   z_vol_load<M, ZV>(u, nl, uu);
-  z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.dto_rho) + e0, nl, dto);
-  z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.absorb) + e0, nl, ab);
+  z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.dto_rho)) + e0, nl, dto);
+  z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.absorb)) + e0, nl, ab);
   z_setup<M, ZV>(twg, rtg, m, a.s, line0, nl, s);
   z_spec_put<M, ZV>(nl, sr, s);
   __syncthreads();
@@ -655,10 +656,11 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   const int ll = threadIdx.x / Z::ZT, t = threadIdx.x % Z::ZT;
   const bool lvalid = ll < nl;
   const long e0 = line0 * M;
-  float2* __restrict__ p = reinterpret_cast<float2*>(m.p) + e0;
+  const gptr<float2> p = G(reinterpret_cast<float2*>(m.p)) + e0;
   float2 pp[CN::EPT];
   if (update) {
-    const int sbeg = m.src_ptr[line0], send = m.src_ptr[line0 + nl];
+    const gptr<const int> sptr = G(m.src_ptr);
+    const int sbeg = sptr[line0], send = sptr[line0 + nl];
     const bool src_fit = send - sbeg <= kSrcCap;  // block-uniform
     int sz_pf[kSrcPT];
     float2 sw_pf[kSrcPT];
@@ -666,17 +668,17 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     for (int q = 0; q < kSrcPT; ++q) {
       const int j = sbeg + threadIdx.x + q * Z::THREADS;
       if (j < send) {
-        sz_pf[q] = m.src_z[j];
-        sw_pf[q] = m.src_ab[j];
+        sz_pf[q] = G(m.src_z)[j];
+        sw_pf[q] = G(m.src_ab)[j];
       }
     }
-    if (threadIdx.x <= nl) s.sptr[threadIdx.x] = m.src_ptr[line0 + threadIdx.x];
+    if (threadIdx.x <= nl) s.sptr[threadIdx.x] = sptr[line0 + threadIdx.x];
     float2 sr[CN::SPT], rc[CN::EPT], ab[CN::EPT], bt[CN::EPT];
-    z_spec_load<M, ZV>(m.sp, line0, nl, a.s, sr);
+    z_spec_load<M, ZV>(G(m.sp), line0, nl, a.s, sr);
     z_vol_load<M, ZV>(p, nl, pp);
-    z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.rhoc2dt) + e0, nl, rc);
-    z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.absorb) + e0, nl, ab);
-    if (m.beta2dt) z_vol_load<M, ZV>(reinterpret_cast<const float2*>(m.beta2dt) + e0, nl, bt);
+    z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.rhoc2dt)) + e0, nl, rc);
+    z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.absorb)) + e0, nl, ab);
+    if (m.beta2dt) z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.beta2dt)) + e0, nl, bt);
     z_setup<M, ZV>(twg, rtg, m, a.s, line0, nl, s);
 #pragma unroll
     for (int q = 0; q < kSrcPT; ++q) {
@@ -689,9 +691,44 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     z_spec_put<M, ZV>(nl, sr, s);
     __syncthreads();
     z_c2r<M, ZV>(ll, t, lvalid, s);
-    const float2* __restrict__ sz = reinterpret_cast<const float2*>(m.sponge);
+    const gptr<const float2> sz = G(reinterpret_cast<const float2*>(m.sponge));
     const int ry = m.rec[3] - m.rec[2], rz = m.rec[5] - m.rec[4];
     unsigned pk = 0u;
+    // store of the finished element (l, mm) at chunk offset i, its convergence peak and,
+    // on a record step, its phasor (Kahan) and p_max updates
+    auto finish = [&](int i, int l, int mm, float2 pv) {
+      const int x = s.lx[l], y = s.ly[l];
+      p[i] = pv;
+      if (x >= m.conv[0] && x < m.conv[1] && y >= m.conv[2] && y < m.conv[3]) {
+        const int z0 = 2 * mm;
+        if (z0 >= m.conv[4] && z0 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.x)));
+        if (z0 + 1 >= m.conv[4] && z0 + 1 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.y)));
+      }
+      if (sm.record && x >= m.rec[0] && x < m.rec[1] && y >= m.rec[2] && y < m.rec[3]) {
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int zz = 2 * mm + e;
+          const float v = e ? pv.y : pv.x;
+          if (zz >= m.rec[4] && zz < m.rec[5]) {
+            const long ri = ((long)(x - m.rec[0]) * ry + (y - m.rec[2])) * rz + (zz - m.rec[4]);
+            for (int h = 0; h < m.nh; ++h) {
+              const gptr<float2> acc = G(m.acc[h]), comp = G(m.comp[h]);
+              const float2 eh = sm.eh[h];
+              const float2 sacc = acc[ri];
+              float2 cp = comp[ri];
+              const float yx = v * eh.x - cp.x, yy = v * eh.y - cp.y;
+              const float tx = sacc.x + yx, ty = sacc.y + yy;
+              cp.x = (tx - sacc.x) - yx;
+              cp.y = (ty - sacc.y) - yy;
+              acc[ri] = make_float2(tx, ty);
+              comp[ri] = cp;
+            }
+            const gptr<float> pmax = G(m.pmax);
+            pmax[ri] = fmaxf(pmax[ri], fabsf(v));
+          }
+        }
+      }
+    };
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
       const int i = threadIdx.x + j * Z::THREADS;
@@ -720,86 +757,28 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
               else pv.x += src_val[e - sbeg];
             }
           }
+          finish(i, l, mm, pv);
         }
         s.a[l * Z::LSA + mm] = pv;
-        if (src_fit) {
-          const int x = s.lx[l], y = s.ly[l];
-          p[i] = pv;
-          if (x >= m.conv[0] && x < m.conv[1] && y >= m.conv[2] && y < m.conv[3]) {
-            const int z0 = 2 * mm;
-            if (z0 >= m.conv[4] && z0 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.x)));
-            if (z0 + 1 >= m.conv[4] && z0 + 1 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.y)));
-          }
-          if (sm.record && x >= m.rec[0] && x < m.rec[1] && y >= m.rec[2] && y < m.rec[3]) {
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-              const int zz = 2 * mm + e;
-              const float v = e ? pv.y : pv.x;
-              if (zz >= m.rec[4] && zz < m.rec[5]) {
-                const long ri = ((long)(x - m.rec[0]) * ry + (y - m.rec[2])) * rz + (zz - m.rec[4]);
-                for (int h = 0; h < m.nh; ++h) {
-                  const float2 eh = sm.eh[h];
-                  float2 sacc = m.acc[h][ri], cp = m.comp[h][ri];
-                  const float yx = v * eh.x - cp.x, yy = v * eh.y - cp.y;
-                  const float tx = sacc.x + yx, ty = sacc.y + yy;
-                  cp.x = (tx - sacc.x) - yx;
-                  cp.y = (ty - sacc.y) - yy;
-                  m.acc[h][ri] = make_float2(tx, ty);
-                  m.comp[h][ri] = cp;
-                }
-                m.pmax[ri] = fmaxf(m.pmax[ri], fabsf(v));
-              }
-            }
-          }
-        }
       }
     }
     if (!src_fit) {
       // more source voxels than the LDS list holds: add them in a separate pass
       __syncthreads();
+      const gptr<const int> srcz = G(m.src_z);
+      const gptr<const float2> srcab = G(m.src_ab);
       for (int j = sbeg + threadIdx.x; j < send; j += Z::THREADS) {
         int l = 0;
         while (l + 1 < nl && s.sptr[l + 1] <= j) ++l;
-        const float2 w = m.src_ab[j];
+        const float2 w = srcab[j];
         float* lf = reinterpret_cast<float*>(s.a + l * Z::LSA);
-        lf[m.src_z[j]] += sm.amp * (w.x * sm.sin_t - w.y * sm.cos_t);
+        lf[srcz[j]] += sm.amp * (w.x * sm.sin_t - w.y * sm.cos_t);
       }
       __syncthreads();
-#pragma unroll
-      for (int j = 0; j < CN::EPT; ++j) {
-        const int i = threadIdx.x + j * Z::THREADS;
-        if (i < nl * M) {
-          const int l = i / M, mm = i - l * M;
-          const int x = s.lx[l], y = s.ly[l];
-          const float2 pv = s.a[l * Z::LSA + mm];
-          p[i] = pv;
-          if (x >= m.conv[0] && x < m.conv[1] && y >= m.conv[2] && y < m.conv[3]) {
-            const int z0 = 2 * mm;
-            if (z0 >= m.conv[4] && z0 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.x)));
-            if (z0 + 1 >= m.conv[4] && z0 + 1 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.y)));
-          }
-          if (sm.record && x >= m.rec[0] && x < m.rec[1] && y >= m.rec[2] && y < m.rec[3]) {
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-              const int zz = 2 * mm + e;
-              const float v = e ? pv.y : pv.x;
-              if (zz >= m.rec[4] && zz < m.rec[5]) {
-                const long ri = ((long)(x - m.rec[0]) * ry + (y - m.rec[2])) * rz + (zz - m.rec[4]);
-                for (int h = 0; h < m.nh; ++h) {
-                  const float2 eh = sm.eh[h];
-                  float2 sacc = m.acc[h][ri], cp = m.comp[h][ri];
-                  const float yx = v * eh.x - cp.x, yy = v * eh.y - cp.y;
-                  const float tx = sacc.x + yx, ty = sacc.y + yy;
-                  cp.x = (tx - sacc.x) - yx;
-                  cp.y = (ty - sacc.y) - yy;
-                  m.acc[h][ri] = make_float2(tx, ty);
-                  m.comp[h][ri] = cp;
-                }
-                m.pmax[ri] = fmaxf(m.pmax[ri], fabsf(v));
-              }
-            }
-          }
-        }
+#pragma unroll 1
+      for (int i = threadIdx.x; i < nl * M; i += Z::THREADS) {
+        const int l = i / M, mm = i - l * M;
+        finish(i, l, mm, s.a[l * Z::LSA + mm]);
       }
     }
     if (sm.peak_idx >= 0) {
@@ -823,7 +802,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     }
   }
   __syncthreads();
-  z_r2c<M, ZV>(ll, t, lvalid, s, m.sp + s.row[lvalid ? ll : 0]);
+  z_r2c<M, ZV>(ll, t, lvalid, s, G(m.sp) + s.row[lvalid ? ll : 0]);
 }
 
 }  // namespace cst

@@ -10,6 +10,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace cst {
 
 constexpr int kMaxBatch = 16;     // members per launch
@@ -47,6 +49,28 @@ struct MemberDev {
   float inv_n;           // 1 / (nx ny nz): the inverse-transform normalization
   int pad_;
 };
+
+// Global-address-space view of a device pointer read from a table. Loads and stores
+// through it compile to global_* instructions; through a generic pointer they become
+// flat_* instructions, which also count against the LDS wait counter, so every wait for
+// an LDS result would first drain the HBM loads in flight.
+// (The host pass only checks kernel bodies; it sees a plain pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+#else
+template <class T>
+using gptr = T*;
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> G(T* p) {
+  return (gptr<T>)p;
+}
+// value of p[i] (a conditional expression cannot mix the qualified lvalue with a float2)
+template <class T>
+__device__ __forceinline__ std::remove_const_t<T> ld(gptr<T> p, long i) {
+  return p[i];
+}
 
 struct StepMember {
   float amp;      // float32(amplitude * ramp envelope at t = n dt)

@@ -139,6 +139,15 @@ int prime(cst_session* s, int slot) {
 
 }  // namespace
 
+namespace cst {
+// plain float4 copy: the calibration kernel of cst_diag_copy
+__global__ void __launch_bounds__(256) k_diag_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                   size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+}  // namespace cst
+
 extern "C" {
 
 int cst_version(void) { return 100; }
@@ -414,6 +423,28 @@ int cst_session_pooled_bytes(cst_session* s, size_t* bytes) {
   size_t n = 0;
   for (auto& b : s->pool) n += b.first;
   *bytes = n;
+  return CST_OK;
+}
+
+int cst_diag_copy(int device, size_t bytes) {
+  if (bytes == 0 || bytes % 16 != 0 || bytes > ((size_t)1 << 34))
+    return fail(CST_ERR_ARG, "bytes must be a positive multiple of 16, at most 16 GiB");
+  HIPCHK(hipSetDevice(device));
+  void* a = nullptr;
+  void* b = nullptr;
+  HIPCHK(hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(CST_ERR_HIP, "hipMalloc failed for the copy target");
+  }
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(cst::k_diag_copy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nullptr,
+                     static_cast<const float4*>(a), static_cast<float4*>(b), n);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (e != hipSuccess) return fail(CST_ERR_HIP, hipGetErrorString(e));
   return CST_OK;
 }
 

@@ -1,0 +1,119 @@
+"""Fixed-length run of one workload's time step, for hardware-counter passes.
+
+``python -m caustica_amd.profile_probe --workload cfg1 --steps 30`` adds the workload
+as one member, advances it ``--steps`` time steps (the last five record), then runs a
+plain copy kernel over a buffer of known size (``cst_diag_copy``: 256 MiB read,
+256 MiB written). Under ``rocprofv3 --pmc FETCH_SIZE`` (or ``WRITE_SIZE``)
+the counter value of that calibration kernel converts the counter's units into bytes,
+which scales the per-dispatch values of the k-space kernels into HBM bytes per launch.
+
+``--summarize-stats DIR`` and ``--summarize-traffic FETCH_DIR WRITE_DIR WORKLOAD`` turn
+rocprofv3 CSV output (``--output-format csv -o run``) into the short text tables kept
+with the measurements; the traffic table ends with ``traffic_bytes_per_launch`` lines
+(pass, workload, bytes) that bench.py reads.
+"""
+
+from __future__ import annotations
+
+import argparse
+import sys
+
+CAL_BYTES = 256 << 20
+
+
+# kernel-name prefix -> pass name used by the session's event timing and bench.py
+PASS_OF = {"cst::k_xgrad": "xgrad", "cst::k_zu": "zu", "cst::k_xdiv": "xdiv", "cst::k_zp": "zp",
+           "cst::k_ypass": "y", "cst::k_finalize": "finalize"}
+
+
+def _short(name: str) -> str:
+    return name.replace("(anonymous namespace)::", "cst::").split("(")[0].replace("void ", "")
+
+
+def summarize_stats(d: str) -> str:
+    """Markdown table of a rocprofv3 --stats kernel summary (run_kernel_stats.csv)."""
+    import csv
+
+    rows = list(csv.DictReader(open(f"{d}/run_kernel_stats.csv")))
+    out = ["| kernel | calls | avg us | min us | max us | share % |", "|---|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        out.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                   f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
+                   f"{float(r['Percentage']):.2f} |")
+    return "\n".join(out)
+
+
+def summarize_traffic(fetch_dir: str, write_dir: str, workload: str) -> str:
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE scaled by the calibration copy kernel."""
+    import csv
+    from collections import defaultdict
+
+    def load(d):
+        agg = defaultdict(list)
+        for r in csv.DictReader(open(f"{d}/run_counter_collection.csv")):
+            agg[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        return {k: (sum(v) / len(v), len(v)) for k, v in agg.items()}
+
+    f, w = load(fetch_dir), load(write_dir)
+    cal = "cst::k_diag_copy"
+    bf, bw = CAL_BYTES / f[cal][0], CAL_BYTES / w[cal][0]
+    out = [f"calibration: {cal} over {CAL_BYTES} B reads FETCH_SIZE {f[cal][0]:.1f} and WRITE_SIZE "
+           f"{w[cal][0]:.1f} -> {bf:.2f} and {bw:.2f} bytes per counter unit", "",
+           "| kernel | dispatches | fetch MB | write MB | total MB |", "|---|---:|---:|---:|---:|"]
+    lines = []
+    for k, (fv, n) in f.items():
+        if k == cal or not k.startswith("cst::"):
+            continue
+        wv = w.get(k, (0.0, 0))[0]
+        tot = fv * bf + wv * bw
+        out.append(f"| `{k}` | {n} | {fv * bf / 1e6:.1f} | {wv * bw / 1e6:.1f} | {tot / 1e6:.1f} |")
+        name = next((v for p, v in PASS_OF.items() if k.startswith(p)), None)
+        if name and name != "y":
+            lines.append(f"traffic_bytes_per_launch {name} {workload} {tot:.0f}")
+    return "\n".join(out + [""] + lines)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--variant", type=int, default=0, help="z-pass thread layout (0 or 1)")
+    ap.add_argument("--summarize-stats", metavar="DIR")
+    ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "WORKLOAD"))
+    args = ap.parse_args()
+    if args.summarize_stats:
+        print(summarize_stats(args.summarize_stats))
+        return 0
+    if args.summarize_traffic:
+        print(summarize_traffic(*args.summarize_traffic))
+        return 0
+
+    from caustica_amd import _native as N
+    from caustica_amd import workloads
+    from caustica_amd.core.backend import get_backend
+    from caustica_amd.solvers.kspace.native import KSpaceSession
+    from caustica_amd.solvers.kspace.operators import plan_member
+
+    get_backend("hip")
+    wl = {"cfg0": workloads.cfg0_water_bowl, "cfg1": workloads.cfg1_hetero_linear,
+          "cfg2": workloads.cfg2_westervelt}[args.workload]()
+    plan = plan_member(wl.grid, wl.medium, wl.source, wl.spec, harmonics=wl.harmonics,
+                       reference_point=wl.reference_point, nonlinear=wl.solver == "westervelt")
+    sess = KSpaceSession(plan.padded3, 1)
+    for p in (2, 3):
+        sess.set_variant(p, args.variant)
+    slot = sess.add(plan, max_periods=args.steps // plan.spp + 4)
+    sess.schedule(slot, max(args.steps - 5, 0))
+    print(f"[probe] {wl.name}: {args.steps} steps, variant {args.variant}", flush=True)
+    sess.advance([slot], args.steps)
+    sess.sync()
+    sess.close()
+
+    print(f"[probe] calibration: copy kernel over {CAL_BYTES} B", flush=True)
+    N.check(N.lib().cst_diag_copy(0, CAL_BYTES))
+    print("[probe] done", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

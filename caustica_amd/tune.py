@@ -3,7 +3,8 @@
 Each tunable pass (0 y, 1 x, 2 zu, 3 zp) has two instantiated thread layouts. For a
 padded shape, ``autotune`` adds a throwaway member (zero state, the caller's plan)
 to the session, times every variant of every pass over a few steps with the
-library's event timing, keeps the faster one, and removes the member. Choices are
+library's event timing (variants interleaved over ``rounds`` rounds, the best round of
+each kept), keeps the faster one, and removes the member. Choices are
 cached per (shape, library version) in a JSON file (``$CAUSTICA_AMD_TUNE_CACHE`` or
 ``~/.cache/caustica_amd/autotune.json``); an unwritable cache only costs re-tuning.
 """
@@ -54,7 +55,7 @@ def apply(sess, choice: dict) -> None:
         sess.set_variant(int(p), int(v))
 
 
-def autotune(sess, plan, steps: int = 6, use_cache: bool = True) -> dict:
+def autotune(sess, plan, steps: int = 6, use_cache: bool = True, rounds: int = 2) -> dict:
     """Pick the fastest variant of each tunable pass for the session's shape."""
     key = _key(sess.padded3)
     if use_cache:
@@ -67,24 +68,25 @@ def autotune(sess, plan, steps: int = 6, use_cache: bool = True) -> dict:
             _memo[key] = choice
             apply(sess, choice)
             return choice
-    slot = sess.add(plan, max_periods=2 + steps // max(plan.spp, 1))
+    slot = sess.add(plan, max_periods=2 + (2 + 2 * rounds * (steps + 1)) * len(PASSES) // max(plan.spp, 1))
     choice = {}
     try:
         sess.advance([slot], 2)  # first-launch costs out of the way
         for p, name in PASSES.items():
-            best = None
-            for v in (0, 1):
-                sess.set_variant(p, v)
-                sess.advance([slot], 1)
-                sess.set_profiling(True)
-                sess.advance([slot], steps)
-                ms, cnt = sess.kernel_times()[name]
-                sess.set_profiling(False)
-                t = ms / max(cnt, 1)
-                if best is None or t < best[0]:
-                    best = (t, v)
-            choice[p] = best[1]
-            sess.set_variant(p, best[1])
+            best = {}
+            # interleaved rounds, best per-launch time of each variant: robust to drift
+            for _ in range(rounds):
+                for v in (0, 1):
+                    sess.set_variant(p, v)
+                    sess.advance([slot], 1)
+                    sess.set_profiling(True)
+                    sess.advance([slot], steps)
+                    ms, cnt = sess.kernel_times()[name]
+                    sess.set_profiling(False)
+                    t = ms / max(cnt, 1)
+                    best[v] = min(best.get(v, t), t)
+            choice[p] = min(best, key=best.get)
+            sess.set_variant(p, choice[p])
     finally:
         sess.remove(slot)
     _memo[key] = choice

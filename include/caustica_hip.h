@@ -101,6 +101,11 @@ int cst_session_trim(cst_session* s, size_t* released);
 int cst_session_pooled_bytes(cst_session* s, size_t* bytes);
 /* Free and total device memory (hipMemGetInfo) of a device, for admission budgets. */
 int cst_device_memory(int device, size_t* free_bytes, size_t* total_bytes);
+/* Calibration kernel for hardware-counter passes (no counterpart in the reference):
+ * allocates two buffers of `bytes` (a multiple of 16), runs one plain copy kernel that
+ * reads `bytes` and writes `bytes`, synchronizes and frees them. A counter's value for
+ * that kernel converts its units into bytes. */
+int cst_diag_copy(int device, size_t bytes);
 /* Zero the member's state, spectra, record sums and peaks and set its step counter to
  * 0, keeping its uploaded maps and source (a fresh solve of the same setup). */
 int cst_member_reset(cst_session* s, int slot);
