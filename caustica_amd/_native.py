@@ -42,6 +42,7 @@ EXPORTS = (
     "cst_member_step",
     "cst_advance",
     "cst_sync",
+    "cst_session_query",
     "cst_member_peaks",
     "cst_member_result",
     "cst_member_get_state",
@@ -143,6 +144,7 @@ def _bind(lib):
         "cst_member_step": ([v, i, C.POINTER(i64)], i),
         "cst_advance": ([v, i, C.POINTER(i), i], i),
         "cst_sync": ([v], i),
+        "cst_session_query": ([v, C.POINTER(i)], i),
         "cst_member_peaks": ([v, i, i, i, p], i),
         "cst_member_result": ([v, i, i, d, p, p], i),
         "cst_member_get_state": ([v, i, p, p, p, p, C.POINTER(i64)], i),

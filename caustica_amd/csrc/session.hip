@@ -528,6 +528,20 @@ int cst_sync(cst_session* s) {
   return CST_OK;
 }
 
+int cst_session_query(cst_session* s, int* idle) {
+  if (!s || !idle) return fail(CST_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(s->device));
+  const hipError_t e = hipStreamQuery(s->stream);
+  if (e == hipSuccess) {
+    *idle = 1;
+  } else if (e == hipErrorNotReady) {
+    *idle = 0;
+  } else {
+    return fail(CST_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+  }
+  return CST_OK;
+}
+
 int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out) {
   if (int r = check_slot(s, slot)) return r;
   Member& m = s->members[slot];

@@ -80,7 +80,9 @@ class _Live:
 @dataclass
 class _Group:
     session: KSpaceSession
+    voxels: int
     live: dict = field(default_factory=dict)  # slot -> _Live
+    busy: bool = False                        # a chunk is queued on the session stream
 
 
 def schedule_of(plan: MemberPlan, spec) -> tuple[int, int]:
@@ -130,8 +132,10 @@ class ContinuousBatcher:
     """Serve a stream of CW solve requests with per-shape batched launches.
 
     hbm_budget: bytes of device memory members may hold at once (default: 80 % of
-    the free device memory at construction). chunk_steps: steps every live member
-    advances between host checks (admission / retirement granularity).
+    the free device memory at construction). chunk_steps: steps one 256^3 member
+    advances per chunk; a group's chunk is scaled to the same voxel-steps (small grids
+    and small batches take more steps per chunk), so every chunk is a few milliseconds
+    of device work and the host revisits a group only when its stream has drained.
     """
 
     def __init__(self, hbm_budget: int | None = None, max_batch: int = 16, chunk_steps: int = 16,
@@ -147,12 +151,14 @@ class ContinuousBatcher:
         self.chunk_steps = int(chunk_steps)
         self.groups: dict[tuple, _Group] = {}
         self.in_use = 0
+        self.peak_in_use = 0  # high-water mark of admitted footprints (bytes)
         self.planners = ThreadPoolExecutor(max_workers=planners)
 
     def _group(self, padded3) -> _Group:
         g = self.groups.get(padded3)
         if g is None:
-            g = _Group(KSpaceSession(padded3, self.max_batch, self.device))
+            g = _Group(KSpaceSession(padded3, self.max_batch, self.device),
+                       int(np.prod(padded3)))
             self.groups[padded3] = g
         return g
 
@@ -212,50 +218,19 @@ class ContinuousBatcher:
                 except Exception as exc:  # noqa: BLE001 - a bad request fails alone
                     responses.append(SolveResponse(req.rid, None, f"{type(exc).__name__}: {exc}",
                                                    req.arrival_s, now(), now(), tuple(req.grid.shape)))
-            # 3. admission by arrival order, HBM budget and free slots
-            still = []
-            for req, plan in waiting:
-                g = self._group(plan.padded3)
-                need = plan.device_bytes()
-                pooled = sum(x.session.pooled_bytes() for x in self.groups.values())
-                if self.in_use + need + pooled > self.hbm_budget and pooled:
-                    for x in self.groups.values():
-                        x.session.trim()  # retired blocks of other shapes give way
-                    pooled = 0
-                if len(g.live) < self.max_batch and self.in_use + need + pooled <= self.hbm_budget:
-                    total, rec = schedule_of(plan, req.spec)
-                    slot = g.session.add(plan, max_periods=plan.eff_max + 4)
-                    g.session.schedule(slot, total - rec)
-                    g.live[slot] = _Live(req, plan, slot, total, rec, now())
-                    self.in_use += need
-                else:
-                    still.append((req, plan))
-            waiting = still
-            # 4. one chunk for every group with live members (async, overlapping streams)
-            active = [g for g in self.groups.values() if g.live]
-            if not active:
-                if nxt < n_total and not waiting and in_flight_plans == 0:
-                    time.sleep(max(0.0, min(pending[nxt].arrival_s - now(), 0.002)))
-                else:
-                    time.sleep(0.0005)
-                continue
-            for g in active:
-                k = min(self.chunk_steps, min(lv.total_steps - lv.done_steps for lv in g.live.values()))
-                g.session.advance(list(g.live), k)
-                for lv in g.live.values():
-                    lv.done_steps += k
-                    lv.peak_batch = max(lv.peak_batch, len(g.live))
-            for g in active:
-                g.session.sync()
-            # 5. retire finished members
-            for g in active:
-                for slot in [s for s, lv in g.live.items() if lv.done_steps >= lv.total_steps]:
+            # 3. groups whose stream drained: retire finished members
+            for g in self.groups.values():
+                if g.busy:
+                    if not g.session.idle():
+                        continue
+                    g.busy = False
+                for slot in [s_ for s_, lv in g.live.items() if lv.done_steps >= lv.total_steps]:
                     lv = g.live.pop(slot)
                     err, res = None, None
                     try:
                         res = finish_result(g.session, slot, lv.plan, lv.req, lv.total_steps,
                                             lv.rec_steps, 0)
-                    except Exception as exc:  # noqa: BLE001
+                    except Exception as exc:  # noqa: BLE001 - a failed solve fails alone
                         err = f"{type(exc).__name__}: {exc}"
                     g.session.remove(slot)
                     self.in_use -= lv.plan.device_bytes()
@@ -264,6 +239,49 @@ class ContinuousBatcher:
                     responses.append(resp)
                     if on_done is not None:
                         on_done(resp)
+            # 4. admission in arrival order, by HBM budget and free slots, into idle groups
+            still = []
+            for req, plan in waiting:
+                g = self._group(plan.padded3)
+                need = plan.device_bytes()
+                pooled = sum(x.session.pooled_bytes() for x in self.groups.values())
+                if self.in_use + need + pooled > self.hbm_budget and pooled:
+                    for x in self.groups.values():
+                        if not x.busy:
+                            x.session.trim()  # retired blocks give way
+                    pooled = sum(x.session.pooled_bytes() for x in self.groups.values())
+                if (not g.busy and len(g.live) < self.max_batch
+                        and self.in_use + need + pooled <= self.hbm_budget):
+                    total, rec = schedule_of(plan, req.spec)
+                    slot = g.session.add(plan, max_periods=plan.eff_max + 4)
+                    g.session.schedule(slot, total - rec)
+                    g.live[slot] = _Live(req, plan, slot, total, rec, now())
+                    self.in_use += need
+                    self.peak_in_use = max(self.peak_in_use, self.in_use)
+                else:
+                    still.append((req, plan))
+            waiting = still
+            # 5. queue the next chunk of every idle group with live members; groups of
+            # different shapes run on their own streams, so their chunks overlap
+            launched = False
+            for g in self.groups.values():
+                if g.busy or not g.live:
+                    continue
+                budget = self.chunk_steps * (256 ** 3)
+                k = max(1, budget // (g.voxels * len(g.live)))
+                k = min(k, min(lv.total_steps - lv.done_steps for lv in g.live.values()))
+                g.session.advance(list(g.live), k)
+                g.busy = True
+                launched = True
+                for lv in g.live.values():
+                    lv.done_steps += k
+                    lv.peak_batch = max(lv.peak_batch, len(g.live))
+            if not launched:
+                if not any(g.busy for g in self.groups.values()) and nxt < n_total \
+                        and not waiting and in_flight_plans == 0:
+                    time.sleep(max(0.0, min(pending[nxt].arrival_s - now(), 0.002)))
+                else:
+                    time.sleep(0.0002)
         return responses
 
 

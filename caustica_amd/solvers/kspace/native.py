@@ -140,6 +140,12 @@ class KSpaceSession:
     def sync(self) -> None:
         N.check(self._lib.cst_sync(self._h))
 
+    def idle(self) -> bool:
+        """True when all work queued on the session's stream has completed (non-blocking)."""
+        v = C.c_int()
+        N.check(self._lib.cst_session_query(self._h, C.byref(v)))
+        return bool(v.value)
+
     def peaks(self, slot: int, first: int, count: int) -> np.ndarray:
         out = np.zeros(max(count, 0), np.float32)
         N.check(self._lib.cst_member_peaks(self._h, int(slot), int(first), int(count), N.ptr(out)))

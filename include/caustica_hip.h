@@ -121,6 +121,10 @@ int cst_member_step(cst_session* s, int slot, int64_t* n);
 int cst_advance(cst_session* s, int n_slots, const int* slots, int n_steps);
 /* Block until the session stream is idle. */
 int cst_sync(cst_session* s);
+/* *idle = 1 when every call queued on the session's stream has completed (non-blocking
+ * hipStreamQuery), 0 while work is in flight: lets one host thread drive several
+ * sessions without waiting on any of them. */
+int cst_session_query(cst_session* s, int* idle);
 
 /* Period peaks (engine.py:490-495): out[i] = max |p| over the convergence box during
  * period first + i (0-based). Synchronizes. NaN/inf propagate (engine.py:137-164). */
