@@ -99,6 +99,77 @@ def cpu_baseline(wl, n_steps: int, workers: int) -> dict:
     }
 
 
+def serving_leg(n_req: int, rates, seed: int, log) -> dict:
+    """Mixed-size queue (128^3, 192^3, 256^3, 128x128x256; 70 % linear / 30 % Westervelt
+    h = 1-2; steered bowls; fresh random media) replayed at Poisson arrivals, served by
+    the continuous batcher and by the serial plain form (one solve at a time through the
+    solver API). Both include host planning, upload and read-back of every request. The
+    same requests are replayed at every rate (arrival times rescaled)."""
+    import dataclasses
+
+    import numpy as np
+
+    from caustica_amd import workloads
+    from caustica_amd.serving import ContinuousBatcher, serve_serial, summarize
+    from caustica_amd.solvers.kspace.native import KSpaceSession
+    from caustica_amd.solvers.kspace.operators import plan_member
+    from caustica_amd.tune import autotune
+
+    log(f"serving: generating {n_req} requests")
+    base = workloads.serving_queue(n_req, rates[0], seed=seed)
+    seen = {}
+    for r in base:
+        seen.setdefault(r.grid.shape, r)
+    for r in seen.values():  # tune every shape once; both forms read the same choice
+        plan = plan_member(r.grid, r.medium, r.source, r.spec, harmonics=r.harmonics,
+                           reference_point=r.reference_point, nonlinear=r.solver == "westervelt")
+        with KSpaceSession(plan.padded3, 2) as sess:
+            autotune(sess, plan)
+
+    def pick(d):
+        return {"solves_per_s": round(d["solves_per_s"], 3),
+                "p50_ms": round(d["latency_p50_s"] * 1e3, 1),
+                "p99_ms": round(d["latency_p99_s"] * 1e3, 1),
+                "makespan_s": round(d["makespan_s"], 2), "errors": d["errors"]}
+
+    runs = []
+    for rate in rates:
+        reqs = [dataclasses.replace(r, arrival_s=r.arrival_s * rates[0] / rate) for r in base]
+        log(f"serving at {rate} req/s: serial plain form")
+        serial = serve_serial(reqs)
+        log(f"serving at {rate} req/s: continuous batcher")
+        b = ContinuousBatcher()
+        try:
+            served = b.serve(reqs)
+        finally:
+            b.close()
+        s_ser, s_srv = summarize(serial), summarize(served)
+        # a single solve: the plain form's service time (start to done) of a 256^3 request
+        single = [r.done_s - r.admit_s for r in serial
+                  if r.shape == (256, 256, 256) and r.error is None]
+        single_ms = float(np.median(single)) * 1e3 if single else None
+        by_rid = {r.rid: r for r in serial}
+        same = all(np.array_equal(r.result.phasor, by_rid[r.rid].result.phasor)
+                   for r in served if r.error is None and by_rid[r.rid].error is None)
+        runs.append({
+            "rate_hz": rate,
+            "served": pick(s_srv) | {"max_batch_seen": s_srv["max_batch_seen"]},
+            "serial_plain": pick(s_ser),
+            "speedup_vs_serial": round(s_srv["solves_per_s"] / max(s_ser["solves_per_s"], 1e-12), 2),
+            "single_solve_256_ms": None if single_ms is None else round(single_ms, 1),
+            "p99_over_single": (None if single_ms is None
+                                else round(s_srv["latency_p99_s"] * 1e3 / single_ms, 2)),
+            "results_bitwise_equal": bool(same),
+        })
+        log(f"serving: {json.dumps(runs[-1])}")
+    return {
+        "queue": {"requests": n_req, "seed": seed,
+                  "shapes": sorted({"x".join(map(str, r.grid.shape)) for r in base}),
+                  "westervelt": sum(r.solver == "westervelt" for r in base)},
+        "runs": runs,
+    }
+
+
 def device_sync(sess):
     sess.sync()
     try:
@@ -119,6 +190,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=5, help="timed CPU baseline steps (0: skip)")
     ap.add_argument("--cpu-workers", type=int, default=1)
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=24,
+                    help="requests of the mixed-size serving queue (0: skip the serving leg)")
+    ap.add_argument("--serving-rates", default="32,2", help="Poisson arrival rates [req/s]")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -219,6 +293,11 @@ def main() -> int:
                             "GB_s": round(step_bytes / (step_ms * 1e-3) / 1e9, 1)}
 
     log(f"gpu: {value:.3f} solves/s; passes {json.dumps(passes)}")
+    sess.close()
+    serving = None
+    if args.serving_requests > 0:
+        rates = [float(v) for v in args.serving_rates.split(",") if v]
+        serving = serving_leg(args.serving_requests, rates, 4, log)
     cpu = None
     if args.cpu_steps > 0:
         log(f"cpu baseline: {args.cpu_steps} steps, workers={args.cpu_workers}")
@@ -252,9 +331,9 @@ def main() -> int:
                        "p99": round(float(np.percentile(lat_ms, 99)), 2)},
         "roofline": roofline,
         "passes": passes,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
-    sess.close()
     print(json.dumps(out))
     return 0
 

@@ -37,14 +37,15 @@ constexpr int cmin(int a, int b) { return a < b ? a : b; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 // L = A * B; stage 1: A DFTs of size B; stage 2: B DFTs of size A.
+// Strided passes: T = max(A, B) threads per line, so every thread holds one DFT per
+// stage (registers: 2 max(A, B) floats per buffer); TC columns per block, at least 16
+// (a 128-byte row segment) and the tile L x TC in LDS at most 64 KiB.
 template <int L>
 struct Plan {
   static constexpr int A = plan_a(L);
   static constexpr int B = L / A;
-  static constexpr int G = cgcd(A, B);
-  static constexpr int T = G >= 4 ? G : B;  // threads per line in the strided passes
-  // strided passes: columns per block (tile of L x TC complex in LDS, <= 48 KiB)
-  static constexpr int TC = cmax(1, pow2_floor(cmin(256 / T, 6144 / L)));
+  static constexpr int T = cmax(A, B);
+  static constexpr int TC = cmax(1, pow2_floor(cmin(cmax(256 / T, 16), 8192 / L)));
 };
 
 // Distribution of one line FFT over T threads.
@@ -229,36 +230,38 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
 
 // ================================================================ x passes
 // Block = (y, column tile of TC bins); element (kx, y, zb) at kx * ps + y * ms + zb.
-template <int L>
-__device__ __forceinline__ void x_kappa(const MemberDev& m, const Shape& s, int y, int zb, int t,
-                                        float (&kap)[Split<L, Plan<L>::T>::R2][Split<L, Plan<L>::T>::A],
-                                        float& kyd, float& kzd) {
-  using S = Split<L, Plan<L>::T>;
-  constexpr int T = Plan<L>::T;
+// Per column (y, zb): kappa / N at x-bin kx is sinc(c_ref dt / 2 |k|) / N with
+// |k|^2 = kx^2 + ky^2 + kz^2; the y and z derivative factors are column constants.
+struct XCol {
+  gptr<const float> kx_full, kx_d;
+  float kyz2, h2, inv_n, kyd, kzd;
+  __device__ __forceinline__ float kappa(int kxi) const {
+    const float kx = kx_full[kxi];
+    return sinc_q(h2 * (kx * kx + kyz2)) * inv_n;
+  }
+};
+
+__device__ __forceinline__ XCol x_col(const MemberDev& m, const Shape& s, int y, int zb) {
   const gptr<const float> kt = G(m.ktab);
-  const gptr<const float> kx_full = kt;
   const gptr<const float> ky_full = kt + 2 * s.nx;
   const gptr<const float> ky_d = ky_full + s.ny;
   const gptr<const float> kz_full = ky_d + s.ny;
   const gptr<const float> kz_d = kz_full + s.nzh;
+  XCol c;
+  c.kx_full = kt;
+  c.kx_d = kt + s.nx;
   const float kyf = ky_full[y], kzf = kz_full[zb];
-  const float kyz2 = kyf * kyf + kzf * kzf;
-  const float h2 = m.half_cdt * m.half_cdt;
-  kyd = ky_d[y];
-  kzd = kz_d[zb];
-#pragma unroll
-  for (int r = 0; r < S::R2; ++r) {
-    const int k2 = t + T * r;
-#pragma unroll
-    for (int k1 = 0; k1 < S::A; ++k1) {
-      const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
-      const float kxf = kx_full[kxi];
-      kap[r][k1] = sinc_q(h2 * (kxf * kxf + kyz2)) * m.inv_n;
-    }
-  }
+  c.kyz2 = kyf * kyf + kzf * kzf;
+  c.h2 = m.half_cdt * m.half_cdt;
+  c.inv_n = m.inv_n;
+  c.kyd = ky_d[y];
+  c.kzd = kz_d[zb];
+  return c;
 }
 
-// gradient: sg[a] = IFFT_x( i k_a kappa / N * FFT_x(sp) ),  a = x, y, z
+// gradient: Q = IFFT_x(i kappa / N FFT_x(sp)); sg[0] = IFFT_x(kx_d i kappa / N FFT_x(sp)),
+// sg[1] = ky_d Q, sg[2] = kz_d Q (ky_d, kz_d are constant along x: one inverse
+// transform serves both).
 template <int L>
 __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     k_xgrad(StepArgs a, const float2* __restrict__ twg) {
@@ -287,43 +290,58 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   __syncthreads();
   float2 X[S::R2][S::A];
   in_finish<L, T, -1>(t, at, X);
-  float kap[S::R2][S::A];
-  float kyd, kzd;
-  x_kappa<L>(m, a.s, y, zb, t, kap, kyd, kzd);
-  const gptr<const float> kx_d = G(m.ktab) + a.s.nx;
+  const XCol c = x_col(m, a.s, y, zb);
+  float2 Z[S::R2][S::A];
 #pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
-    float2 Z[S::R2][S::A];
+  for (int r = 0; r < S::R2; ++r) {
+    const int k2 = t + T * r;
 #pragma unroll
-    for (int r = 0; r < S::R2; ++r) {
-      const int k2 = t + T * r;
-#pragma unroll
-      for (int k1 = 0; k1 < S::A; ++k1) {
-        const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
-        const float kd = ax == 0 ? kx_d[kxi] : (ax == 1 ? kyd : kzd);
-        const float d = kd * kap[r][k1];
-        Z[r][k1] = make_float2(-d * X[r][k1].y, d * X[r][k1].x);
-      }
+    for (int k1 = 0; k1 < S::A; ++k1) {
+      const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
+      const float kp = c.kappa(kxi);
+      X[r][k1] = make_float2(-kp * X[r][k1].y, kp * X[r][k1].x);  // i kappa / N X
+      const float kd = c.kx_d[kxi];
+      Z[r][k1] = make_float2(kd * X[r][k1].x, kd * X[r][k1].y);
     }
+  }
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
     __syncthreads();
-    out_compute<L, T, +1>(t, Z, at, tw);
+    out_compute<L, T, +1>(t, pass == 0 ? Z : X, at, tw);
     __syncthreads();
     out_finish<L, T, +1>(t, at, V);
     if (valid) {
-      const gptr<float2> dst = G(m.sg[ax]);
+      if (pass == 0) {
+        const gptr<float2> dst = G(m.sg[0]);
 #pragma unroll
-      for (int r = 0; r < S::R1; ++r) {
-        const int m2 = t + T * r;
-        if (m2 < S::A) {
+        for (int r = 0; r < S::R1; ++r) {
+          const int m2 = t + T * r;
+          if (m2 < S::A) {
 #pragma unroll
-          for (int j = 0; j < S::B; ++j) dst[base + (long)(m2 + S::A * j) * C] = V[r][j];
+            for (int j = 0; j < S::B; ++j) dst[base + (long)(m2 + S::A * j) * C] = V[r][j];
+          }
+        }
+      } else {
+        const gptr<float2> dy = G(m.sg[1]), dz = G(m.sg[2]);
+#pragma unroll
+        for (int r = 0; r < S::R1; ++r) {
+          const int m2 = t + T * r;
+          if (m2 < S::A) {
+#pragma unroll
+            for (int j = 0; j < S::B; ++j) {
+              const long o = base + (long)(m2 + S::A * j) * C;
+              dy[o] = make_float2(c.kyd * V[r][j].x, c.kyd * V[r][j].y);
+              dz[o] = make_float2(c.kzd * V[r][j].x, c.kzd * V[r][j].y);
+            }
+          }
         }
       }
     }
   }
 }
 
-// divergence: sp = IFFT_x( sum_a i k_a kappa / N * FFT_x(sg[a]) )
+// divergence: sp = IFFT_x( i kappa / N (kx_d FFT_x(sg[0]) + FFT_x(ky_d sg[1] + kz_d sg[2])) )
+// (the y and z factors are constant along x, so their two spectra share one transform)
 template <int L>
 __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     k_xdiv(StepArgs a, const float2* __restrict__ twg) {
@@ -343,45 +361,42 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const int zb = valid ? zb0 : 0;
   const long C = a.s.ps;
   const long base = (long)y * a.s.ms + zb;
-  float kap[S::R2][S::A];
-  float kyd, kzd;
-  x_kappa<L>(m, a.s, y, zb, t, kap, kyd, kzd);
-  const gptr<const float> kx_d = G(m.ktab) + a.s.nx;
+  float2 V[S::R1][S::B], Wy[S::R1][S::B], Wz[S::R1][S::B];
+  const gptr<const float2> ux = G(m.sg[0]), uy = G(m.sg[1]), uz = G(m.sg[2]);
+  in_load<L, T>(t, [&](int n) { return valid ? ld(ux, base + n * C) : make_float2(0.f, 0.f); }, V);
+  in_load<L, T>(t, [&](int n) { return valid ? ld(uy, base + n * C) : make_float2(0.f, 0.f); }, Wy);
+  in_load<L, T>(t, [&](int n) { return valid ? ld(uz, base + n * C) : make_float2(0.f, 0.f); }, Wz);
+  const XCol c = x_col(m, a.s, y, zb);
   TileAt<TC> at{tile, col};
-  float2 Sacc[S::R2][S::A];
-  float2 V[S::R1][S::B], Vn[S::R1][S::B];
-  const gptr<const float2> src0 = G(m.sg[0]);
-  in_load<L, T>(t, [&](int n) { return valid ? ld(src0, base + n * C) : make_float2(0.f, 0.f); }, Vn);
+  __syncthreads();
+  in_compute<L, T, -1>(t, V, at, tw);
+  __syncthreads();
+  float2 Xx[S::R2][S::A];
+  in_finish<L, T, -1>(t, at, Xx);
 #pragma unroll
-  for (int ax = 0; ax < 3; ++ax) {
+  for (int r = 0; r < S::R1; ++r)
 #pragma unroll
-    for (int r = 0; r < S::R1; ++r)
+    for (int i = 0; i < S::B; ++i)
+      V[r][i] = make_float2(c.kyd * Wy[r][i].x + c.kzd * Wz[r][i].x, c.kyd * Wy[r][i].y + c.kzd * Wz[r][i].y);
+  __syncthreads();
+  in_compute<L, T, -1>(t, V, at, tw);
+  __syncthreads();
+  float2 X[S::R2][S::A];
+  in_finish<L, T, -1>(t, at, X);
 #pragma unroll
-      for (int i = 0; i < S::B; ++i) V[r][i] = Vn[r][i];
-    if (ax < 2) {  // the next axis' column is in flight during this axis' transform
-      const gptr<const float2> nsrc = G(m.sg[ax + 1]);
-      in_load<L, T>(t, [&](int n) { return valid ? ld(nsrc, base + n * C) : make_float2(0.f, 0.f); }, Vn);
-    }
-    __syncthreads();
-    in_compute<L, T, -1>(t, V, at, tw);
-    __syncthreads();
-    float2 X[S::R2][S::A];
-    in_finish<L, T, -1>(t, at, X);
+  for (int r = 0; r < S::R2; ++r) {
+    const int k2 = t + T * r;
 #pragma unroll
-    for (int r = 0; r < S::R2; ++r) {
-      const int k2 = t + T * r;
-#pragma unroll
-      for (int k1 = 0; k1 < S::A; ++k1) {
-        const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
-        const float kd = ax == 0 ? kx_d[kxi] : (ax == 1 ? kyd : kzd);
-        const float d = kd * kap[r][k1];
-        const float2 term = make_float2(-d * X[r][k1].y, d * X[r][k1].x);
-        Sacc[r][k1] = ax == 0 ? term : cadd(Sacc[r][k1], term);
-      }
+    for (int k1 = 0; k1 < S::A; ++k1) {
+      const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
+      const float kd = c.kx_d[kxi];
+      const float kp = c.kappa(kxi);
+      const float2 sum = make_float2(kd * Xx[r][k1].x + X[r][k1].x, kd * Xx[r][k1].y + X[r][k1].y);
+      X[r][k1] = make_float2(-kp * sum.y, kp * sum.x);
     }
   }
   __syncthreads();
-  out_compute<L, T, +1>(t, Sacc, at, tw);
+  out_compute<L, T, +1>(t, X, at, tw);
   __syncthreads();
   out_finish<L, T, +1>(t, at, V);
   if (valid) {

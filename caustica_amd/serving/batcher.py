@@ -154,19 +154,24 @@ class ContinuousBatcher:
         self.peak_in_use = 0  # high-water mark of admitted footprints (bytes)
         self.planners = ThreadPoolExecutor(max_workers=planners)
 
-    def _group(self, padded3) -> _Group:
-        g = self.groups.get(padded3)
+    def _group(self, plan: MemberPlan) -> _Group:
+        g = self.groups.get(plan.padded3)
         if g is None:
-            g = _Group(KSpaceSession(padded3, self.max_batch, self.device),
-                       int(np.prod(padded3)))
-            self.groups[padded3] = g
+            from caustica_amd.solvers.kspace.engine import TUNE_MIN_VOXELS
+            from caustica_amd.tune import autotune
+
+            g = _Group(KSpaceSession(plan.padded3, self.max_batch, self.device),
+                       int(np.prod(plan.padded3)))
+            # the same per-shape variant choice as the plain solve path (cached), so a
+            # served result is bit-identical to the same solve run alone
+            if g.voxels >= TUNE_MIN_VOXELS:
+                autotune(g.session, plan)
+            self.groups[plan.padded3] = g
         return g
 
     def tune(self, plan: MemberPlan) -> None:
-        """Autotune the pass variants of a shape group before serving (cached per shape)."""
-        from caustica_amd.tune import autotune
-
-        autotune(self._group(plan.padded3).session, plan)
+        """Create (and autotune) the shape group of ``plan`` ahead of the first request."""
+        self._group(plan)
 
     @staticmethod
     def _plan(req: SolveRequest) -> MemberPlan:
@@ -242,7 +247,7 @@ class ContinuousBatcher:
             # 4. admission in arrival order, by HBM budget and free slots, into idle groups
             still = []
             for req, plan in waiting:
-                g = self._group(plan.padded3)
+                g = self._group(plan)
                 need = plan.device_bytes()
                 pooled = sum(x.session.pooled_bytes() for x in self.groups.values())
                 if self.in_use + need + pooled > self.hbm_budget and pooled:
