@@ -197,7 +197,7 @@ def main() -> int:
     if args.gpus != 1:
         print(
             "bench.py runs on one GPU only: this build targets a single MI355X; independent "
-            "solves shard across GPUs with no exchange (see DESIGN.md), so no multi-GPU "
+            "solves shard across GPUs with no exchange (one batcher per GPU), so no multi-GPU "
             "figure is produced.",
             file=sys.stderr,
         )

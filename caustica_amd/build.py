@@ -24,7 +24,7 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    # packed fp32 VALU forms stay off ("code generation")
+    # no packed FP32 VALU forms (v_pk_{add,mul,fma}_f32) anywhere in the device code
     "-Xclang",
     "-target-feature",
     "-Xclang",

@@ -454,6 +454,7 @@ struct ZLds {
   float2 e[ZPlan<M, ZV>::LPB * ZPlan<M, ZV>::LSE];
   float2 tw[M];
   float2 rt[M + 1];
+  float2 sz[M];  // z sponge profile as pairs (z = 2 mm, 2 mm + 1)
   float sxy[ZPlan<M, ZV>::LPB];
   int lx[ZPlan<M, ZV>::LPB];
   int ly[ZPlan<M, ZV>::LPB];
@@ -473,16 +474,57 @@ __device__ __forceinline__ long row_of(long line, const Shape& s) {
   return x * s.ps + (line - x * s.ny) * s.ms;
 }
 
+// Tables a block stages into LDS (twiddles, real-split factors, z sponge, per-line
+// sponge product and spectrum-row offsets). They are loaded into registers first and
+// written to LDS after the block's HBM loads are issued: a wait for a load also waits
+// for every load issued before it, so anything consumed early must be loaded early.
 template <int M, int ZV>
-__device__ __forceinline__ void z_setup(const float2* twg, const float2* rtg, const MemberDev& m,
-                                        const Shape& sh, long line0, int nl, ZLds<M, ZV>& s) {
-  for (int i = threadIdx.x; i < M; i += ZPlan<M, ZV>::THREADS) s.tw[i] = twg[i];
-  for (int i = threadIdx.x; i <= M; i += ZPlan<M, ZV>::THREADS) s.rt[i] = rtg[i];
+struct ZStage {
+  static constexpr int THREADS = ZPlan<M, ZV>::THREADS;
+  static constexpr int TPT = (M + 1 + THREADS - 1) / THREADS;
+  float2 tw[TPT], rt[TPT], sz[TPT];
+  float sx, sy;
+};
+
+template <int M, int ZV>
+__device__ __forceinline__ void z_stage_load(const float2* twg, const float2* rtg, const MemberDev& m,
+                                             const Shape& sh, long line0, int nl, ZStage<M, ZV>& st) {
+  // every load is unconditional (indices clamped into range): the wait-count pass can
+  // then wait for exactly the loads a consumer needs instead of for all of them
+  constexpr int TH = ZStage<M, ZV>::THREADS;
+  const gptr<const float2> szg = G(reinterpret_cast<const float2*>(m.sponge));
+#pragma unroll
+  for (int q = 0; q < ZStage<M, ZV>::TPT; ++q) {
+    const int i = threadIdx.x + q * TH;
+    const int im = i < M ? i : M - 1;
+    st.tw[q] = twg[im];
+    st.sz[q] = szg[im];
+    st.rt[q] = rtg[i <= M ? i : M];
+  }
+  const int line = (int)line0 + ((int)threadIdx.x < nl ? (int)threadIdx.x : nl - 1);
+  const int x = line / sh.ny, y = line - x * sh.ny;
+  const gptr<const float> sxg = G(m.sponge) + 2 * M;
+  st.sx = sxg[x];
+  st.sy = sxg[sh.nx + y];
+}
+
+template <int M, int ZV>
+__device__ __forceinline__ void z_stage_put(const ZStage<M, ZV>& st, const Shape& sh, long line0, int nl,
+                                            ZLds<M, ZV>& s) {
+  constexpr int TH = ZStage<M, ZV>::THREADS;
+#pragma unroll
+  for (int q = 0; q < ZStage<M, ZV>::TPT; ++q) {
+    const int i = threadIdx.x + q * TH;
+    if (i < M) {
+      s.tw[i] = st.tw[q];
+      s.sz[i] = st.sz[q];
+    }
+    if (i <= M) s.rt[i] = st.rt[q];
+  }
   if (threadIdx.x < nl) {
     const int line = (int)line0 + threadIdx.x;
     const int x = line / sh.ny, y = line - x * sh.ny;
-    const gptr<const float> sx = G(m.sponge) + 2 * M;
-    s.sxy[threadIdx.x] = sx[x] * sx[sh.nx + y];
+    s.sxy[threadIdx.x] = st.sx * st.sy;
     s.lx[threadIdx.x] = x;
     s.ly[threadIdx.x] = y;
     s.row[threadIdx.x] = (long)x * sh.ps + (long)y * sh.ms;
@@ -495,19 +537,23 @@ __device__ __forceinline__ void z_spec_load(gptr<const float2> spec, long line0,
                                             const Shape& sh, float2 (&r)[ZCount<M, ZV>::SPT]) {
   constexpr int NB = M + 1;
   const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;
+  // lines of a block are consecutive; with LPB <= ny they cross at most one x plane
+  const bool one_wrap = ZPlan<M, ZV>::LPB <= sh.ny;
 #pragma unroll
   for (int j = 0; j < ZCount<M, ZV>::SPT; ++j) {
-    const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
-    if (i < nl * NB) {
-      const int l = i / NB, k = i - l * NB;
-      // lines of a block are consecutive; they cross at most a few x planes
-      int y = y0 + l, x = x0;
-      while (y >= sh.ny) {
-        y -= sh.ny;
-        ++x;
-      }
-      r[j] = spec[(long)x * sh.ps + (long)y * sh.ms + k];
+    const int i0 = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
+    const int i = i0 < nl * NB ? i0 : nl * NB - 1;  // unconditional load (see z_stage_load)
+    const int l = i / NB, k = i - l * NB;
+    int y = y0 + l, x = x0;
+    if (one_wrap) {
+      const bool w = y >= sh.ny;
+      y -= w ? sh.ny : 0;
+      x += w ? 1 : 0;
+    } else {
+      x += y / sh.ny;
+      y -= (y / sh.ny) * sh.ny;
     }
+    r[j] = spec[(long)x * sh.ps + (long)y * sh.ms + k];
   }
 }
 
@@ -531,7 +577,7 @@ __device__ __forceinline__ void z_vol_load(gptr<const float2> v, int nl,
 #pragma unroll
   for (int j = 0; j < ZCount<M, ZV>::EPT; ++j) {
     const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
-    r[j] = (i < nl * M) ? ld(v, i) : make_float2(0.f, 0.f);
+    r[j] = ld(v, i < nl * M ? i : nl * M - 1);  // unconditional load (see z_stage_load)
   }
 }
 
@@ -614,15 +660,18 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   const int ll = threadIdx.x / Z::ZT, t = threadIdx.x % Z::ZT;
   const bool lvalid = ll < nl;
   const long e0 = line0 * M;  // first float2 of the block's chunk
-  const gptr<const float2> sz = G(reinterpret_cast<const float2*>(m.sponge));
   const gptr<float2> spec = G(m.sg[ax]);
   const gptr<float2> u = G(reinterpret_cast<float2*>(m.u[ax])) + e0;
+  // loads in the order they are consumed: staged tables, spectrum (before the C2R),
+  // then the volumes (after it: their latency overlaps the transform)
+  ZStage<M, ZV> stg;
+  z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
   float2 sr[CN::SPT], uu[CN::EPT], dto[CN::EPT], ab[CN::EPT];
   z_spec_load<M, ZV>(spec, line0, nl, a.s, sr);
   z_vol_load<M, ZV>(u, nl, uu);
   z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.dto_rho)) + e0, nl, dto);
   z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.absorb)) + e0, nl, ab);
-  z_setup<M, ZV>(twg, rtg, m, a.s, line0, nl, s);
+  z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
   z_spec_put<M, ZV>(nl, sr, s);
   __syncthreads();
   z_c2r<M, ZV>(ll, t, lvalid, s);
@@ -633,7 +682,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
       const int l = i / M, mm = i - l * M;
       const float sxy = s.sxy[l];
       const float2 gg = s.a[l * Z::LSA + mm];
-      const float2 s2 = sz[mm];
+      const float2 s2 = s.sz[mm];
       float2 uv = uu[j];
       uv.x -= dto[j].x * gg.x;
       uv.y -= dto[j].y * gg.y;
@@ -658,7 +707,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     k_zp(StepArgs a, const float2* __restrict__ twg, const float2* __restrict__ rtg, int update) {
   using Z = ZPlan<M, ZV>;
   using CN = ZCount<M, ZV>;
-  constexpr int kSrcPT = 2;                       // prefetched source entries per thread
+  constexpr int kSrcPT = 1;                       // prefetched source entries per thread
   constexpr int kSrcCap = kSrcPT * Z::THREADS;    // entries a block keeps in LDS
   __shared__ ZLds<M, ZV> s;
   __shared__ int src_zl[kSrcCap];
@@ -674,27 +723,33 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   const gptr<float2> p = G(reinterpret_cast<float2*>(m.p)) + e0;
   float2 pp[CN::EPT];
   if (update) {
+    // loads in the order they are consumed: staged tables, spectrum, source list (all
+    // before the C2R), then the volumes (after it: their latency overlaps the transform)
+    ZStage<M, ZV> stg;
+    z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
+    float2 sr[CN::SPT], rc[CN::EPT], ab[CN::EPT], bt[CN::EPT];
+    z_spec_load<M, ZV>(G(m.sp), line0, nl, a.s, sr);
     const gptr<const int> sptr = G(m.src_ptr);
     const int sbeg = sptr[line0], send = sptr[line0 + nl];
     const bool src_fit = send - sbeg <= kSrcCap;  // block-uniform
     int sz_pf[kSrcPT];
     float2 sw_pf[kSrcPT];
+    int sp_l = 0;
+    // unconditional loads (see z_stage_load); the source arrays hold at least one entry
 #pragma unroll
     for (int q = 0; q < kSrcPT; ++q) {
       const int j = sbeg + threadIdx.x + q * Z::THREADS;
-      if (j < send) {
-        sz_pf[q] = G(m.src_z)[j];
-        sw_pf[q] = G(m.src_ab)[j];
-      }
+      const int jc = j < send ? j : (send > 0 ? send - 1 : 0);
+      sz_pf[q] = G(m.src_z)[jc];
+      sw_pf[q] = G(m.src_ab)[jc];
     }
-    if (threadIdx.x <= nl) s.sptr[threadIdx.x] = sptr[line0 + threadIdx.x];
-    float2 sr[CN::SPT], rc[CN::EPT], ab[CN::EPT], bt[CN::EPT];
-    z_spec_load<M, ZV>(G(m.sp), line0, nl, a.s, sr);
+    sp_l = sptr[line0 + ((int)threadIdx.x <= nl ? (int)threadIdx.x : nl)];
     z_vol_load<M, ZV>(p, nl, pp);
     z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.rhoc2dt)) + e0, nl, rc);
     z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.absorb)) + e0, nl, ab);
     if (m.beta2dt) z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.beta2dt)) + e0, nl, bt);
-    z_setup<M, ZV>(twg, rtg, m, a.s, line0, nl, s);
+    z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
+    if (threadIdx.x <= nl) s.sptr[threadIdx.x] = sp_l;
 #pragma unroll
     for (int q = 0; q < kSrcPT; ++q) {
       const int j = sbeg + threadIdx.x + q * Z::THREADS;
@@ -706,7 +761,6 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     z_spec_put<M, ZV>(nl, sr, s);
     __syncthreads();
     z_c2r<M, ZV>(ll, t, lvalid, s);
-    const gptr<const float2> sz = G(reinterpret_cast<const float2*>(m.sponge));
     const int ry = m.rec[3] - m.rec[2], rz = m.rec[5] - m.rec[4];
     unsigned pk = 0u;
     // store of the finished element (l, mm) at chunk offset i, its convergence peak and,
@@ -757,7 +811,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
           c.x += bt[j].x * pv.x;
           c.y += bt[j].y * pv.y;
         }
-        const float2 s2 = sz[mm];
+        const float2 s2 = s.sz[mm];
         pv.x -= c.x * dv.x;
         pv.y -= c.y * dv.y;
         pv.x *= ab[j].x;
@@ -805,8 +859,10 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
       }
     }
   } else {
+    ZStage<M, ZV> stg;
+    z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
     z_vol_load<M, ZV>(p, nl, pp);
-    z_setup<M, ZV>(twg, rtg, m, a.s, line0, nl, s);
+    z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
       const int i = threadIdx.x + j * Z::THREADS;

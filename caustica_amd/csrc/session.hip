@@ -304,8 +304,9 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   const size_t o_k = c.take(nk * 4);
   const size_t o_sp = c.take(nsp * 4);
   const size_t o_ptr = c.take((nl + 1) * 4);
-  const size_t o_sz = c.take(src_z.size() * 4);
-  const size_t o_sab = c.take(src_ab.size() * 8);
+  // at least one entry each: the z pass loads index 0 unconditionally
+  const size_t o_sz = c.take((int64_t)std::max<size_t>(src_z.size(), 1) * 4);
+  const size_t o_sab = c.take((int64_t)std::max<size_t>(src_ab.size(), 1) * 8);
   const size_t o_rec = c.take(2 * d->n_harmonics * R * 8);
   const size_t o_pmax = c.take(R * 4);
   const size_t o_peaks = c.take((int64_t)maxp * kPeakWords * 4);

@@ -1,7 +1,8 @@
 """Fixed-length run of one workload's time step, for hardware-counter passes.
 
 ``python -m caustica_amd.profile_probe --workload cfg1 --steps 30`` adds the workload
-as one member, advances it ``--steps`` time steps (the last five record), then runs a
+as one member, advances it ``--steps`` time steps (the last ``--record-steps``, default
+5, record), then runs a
 plain copy kernel over a buffer of known size (``cst_diag_copy``: 256 MiB read,
 256 MiB written). Under ``rocprofv3 --pmc FETCH_SIZE`` (or ``WRITE_SIZE``)
 the counter value of that calibration kernel converts the counter's units into bytes,
@@ -78,6 +79,7 @@ def main() -> int:
     ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--variant", type=int, default=0, help="z-pass thread layout (0 or 1)")
+    ap.add_argument("--record-steps", type=int, default=5, help="trailing steps that record")
     ap.add_argument("--summarize-stats", metavar="DIR")
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "WORKLOAD"))
     args = ap.parse_args()
@@ -103,8 +105,10 @@ def main() -> int:
     for p in (2, 3):
         sess.set_variant(p, args.variant)
     slot = sess.add(plan, max_periods=args.steps // plan.spp + 4)
-    sess.schedule(slot, max(args.steps - 5, 0))
-    print(f"[probe] {wl.name}: {args.steps} steps, variant {args.variant}", flush=True)
+    if args.record_steps > 0:
+        sess.schedule(slot, max(args.steps - args.record_steps, 0))
+    print(f"[probe] {wl.name}: {args.steps} steps ({args.record_steps} recording), "
+          f"variant {args.variant}", flush=True)
     sess.advance([slot], args.steps)
     sess.sync()
     sess.close()
