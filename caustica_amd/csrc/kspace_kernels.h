@@ -84,9 +84,15 @@ __device__ __forceinline__ void in_load(int t, Load load, float2 (&V)[Split<L, T
 #pragma unroll
   for (int r = 0; r < S::R1; ++r) {
     const int n1 = t + T * r;
+    if constexpr (T * S::R1 == S::A) {
+      // every thread owns a sub-line: unconditional loads keep the wait counts exact
 #pragma unroll
-    for (int n2 = 0; n2 < S::B; ++n2)
-      V[r][n2] = (n1 < S::A) ? load(n1 + S::A * n2) : make_float2(0.f, 0.f);
+      for (int n2 = 0; n2 < S::B; ++n2) V[r][n2] = load(n1 + S::A * n2);
+    } else {
+#pragma unroll
+      for (int n2 = 0; n2 < S::B; ++n2)
+        V[r][n2] = (n1 < S::A) ? load(n1 + S::A * n2) : make_float2(0.f, 0.f);
+    }
   }
 }
 
@@ -173,6 +179,28 @@ struct LinePadAt {
   __device__ __forceinline__ float2* operator()(int row) const { return base + row + row / PAD; }
 };
 
+// Twiddle table of a strided pass, staged through registers: written to LDS after the
+// column loads are issued, so those are in flight while the block waits for the table.
+template <int L, int NT>
+struct TwStage {
+  static constexpr int Q = (L + NT - 1) / NT;
+  float2 v[Q];
+  __device__ __forceinline__ void load(const float2* __restrict__ twg) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int i = threadIdx.x + q * NT;
+      v[q] = twg[i < L ? i : L - 1];
+    }
+  }
+  __device__ __forceinline__ void put(float2* tw) const {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int i = threadIdx.x + q * NT;
+      if (i < L) tw[i] = v[q];
+    }
+  }
+};
+
 // sin(x) / x as its Taylor series in q = x^2 (alternating terms (-q)^n / (2n+1)!, up to
 // n = 7): for the k-space correction x = (c_ref dt / 2) |k| <= (pi / 2) CFL sqrt(3) <= 1.64
 // (CFL <= 0.6), so q <= 2.7 and the first omitted term is below 1e-12.
@@ -198,7 +226,8 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   constexpr int T = P::T;
   __shared__ float2 tile[L * TC];
   __shared__ float2 tw[L];
-  for (int i = threadIdx.x; i < L; i += TC * T) tw[i] = twg[i];
+  TwStage<L, TC * T> tws;
+  tws.load(twg);
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
   const gptr<float2> data = G(which == 0 ? m.sp : m.sg[blockIdx.y]);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
@@ -208,9 +237,12 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const bool valid = zb < a.s.nzh;
   const long S_ = a.s.ms;
   const long base = (long)x * a.s.ps + (valid ? zb : 0);
-  __syncthreads();
   float2 V[S::R1][S::B];
-  in_load<L, T>(t, [&](int n) { return valid ? ld(data, base + n * S_) : make_float2(0.f, 0.f); }, V);
+  // invalid columns load bin 0 of their row (a valid address) and are never stored:
+  // unconditional loads let the wait-count pass wait for exactly what is consumed
+  in_load<L, T>(t, [&](int n) { return ld(data, base + n * S_); }, V);
+  tws.put(tw);
+  __syncthreads();
   TileAt<TC> at{tile, col};
   in_compute<L, T, DIR>(t, V, at, tw);
   __syncthreads();
@@ -271,7 +303,8 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   constexpr int TC = P::TC;
   __shared__ float2 tile[L * TC];
   __shared__ float2 tw[L];
-  for (int i = threadIdx.x; i < L; i += TC * T) tw[i] = twg[i];
+  TwStage<L, TC * T> tws;
+  tws.load(twg);
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
   const int nt = (a.s.nzh + TC - 1) / TC;
@@ -281,10 +314,11 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const int zb = valid ? zb0 : 0;
   const long C = a.s.ps;
   const long base = (long)y * a.s.ms + zb;
-  __syncthreads();
   float2 V[S::R1][S::B];
   const gptr<const float2> src = G(m.sp);
-  in_load<L, T>(t, [&](int n) { return valid ? ld(src, base + n * C) : make_float2(0.f, 0.f); }, V);
+  in_load<L, T>(t, [&](int n) { return ld(src, base + n * C); }, V);  // see k_ypass
+  tws.put(tw);
+  __syncthreads();
   TileAt<TC> at{tile, col};
   in_compute<L, T, -1>(t, V, at, tw);
   __syncthreads();
@@ -351,7 +385,8 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   constexpr int TC = P::TC;
   __shared__ float2 tile[L * TC];
   __shared__ float2 tw[L];
-  for (int i = threadIdx.x; i < L; i += TC * T) tw[i] = twg[i];
+  TwStage<L, TC * T> tws;
+  tws.load(twg);
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
   const int nt = (a.s.nzh + TC - 1) / TC;
@@ -363,12 +398,15 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const long base = (long)y * a.s.ms + zb;
   float2 V[S::R1][S::B], Wy[S::R1][S::B], Wz[S::R1][S::B];
   const gptr<const float2> ux = G(m.sg[0]), uy = G(m.sg[1]), uz = G(m.sg[2]);
-  in_load<L, T>(t, [&](int n) { return valid ? ld(ux, base + n * C) : make_float2(0.f, 0.f); }, V);
-  in_load<L, T>(t, [&](int n) { return valid ? ld(uy, base + n * C) : make_float2(0.f, 0.f); }, Wy);
-  in_load<L, T>(t, [&](int n) { return valid ? ld(uz, base + n * C) : make_float2(0.f, 0.f); }, Wz);
+  in_load<L, T>(t, [&](int n) { return ld(ux, base + n * C); }, V);  // see k_ypass
   const XCol c = x_col(m, a.s, y, zb);
   TileAt<TC> at{tile, col};
+  tws.put(tw);
   __syncthreads();
+  // issued after the barrier: the first transform then waits for U_x only, and these
+  // loads overlap it
+  in_load<L, T>(t, [&](int n) { return ld(uy, base + n * C); }, Wy);
+  in_load<L, T>(t, [&](int n) { return ld(uz, base + n * C); }, Wz);
   in_compute<L, T, -1>(t, V, at, tw);
   __syncthreads();
   float2 Xx[S::R2][S::A];
