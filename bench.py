@@ -99,6 +99,55 @@ def cpu_baseline(wl, n_steps: int, workers: int) -> dict:
     }
 
 
+def thermal_leg(sess, slot, plan, wl, rec_steps: int, n_heat: int, n_cool: int, log) -> dict:
+    """Pennes + CEM43 on the 256^3 grid, heated by the resident 256^3 phasor: the heat
+    source is formed on the device (alpha |P|^2 / (rho c)), scaled to 150 W/cm^2 focal
+    intensity; dt 0.05 s, n_heat heating steps then n_cool cooling steps, perfused tissue
+    (k 0.5, rho C 3.6e6, w_b 0.5 kg/m^3/s), T0 = T_art = 37 C. Roofline: 48 P bytes per
+    heating step and 44 P per cooling step (theta, Q, k, 1/(rho C) dt, perfusion read,
+    theta written, theta_max and the float64 dose read and written; no Q when cooling)."""
+    import numpy as np
+
+    from caustica_amd import workloads
+    from caustica_amd.sensors import DeviceHeatingSource
+    from caustica_amd.thermal import PennesSolver, ThermalResult
+
+    shape = plan.rec_shape
+    P = int(np.prod(shape))
+    focus = tuple(int(v) for v in wl.reference_point)
+    ph = sess.result(slot, 0, 2.0 / rec_steps, with_pmax=False)[0].reshape(shape)
+    scale = workloads.focal_intensity_scale(ph[focus], float(wl.medium.rho[focus]),
+                                            float(wl.medium.c[focus]))
+    src = DeviceHeatingSource.from_member(sess, slot, plan, wl.medium, wl.grid.dx,
+                                          rec_steps=rec_steps).scaled(scale)
+    med = workloads.pennes_tissue(shape, wl.grid.dx)
+    ps = PennesSolver("hip", profile=True)
+    t0 = np.full(shape, 37.0, np.float32)
+    log(f"thermal: {n_heat} heating + {n_cool} cooling steps on {shape}")
+    w0 = time.perf_counter()
+    hot = ps.solve(t0, src, med, dt=0.05, n_steps=n_heat, dose=True)
+    cool = ps.solve(hot.temperature, None, med, dt=0.05, n_steps=n_cool, dose=True,
+                    dose0=hot.dose_cem43)
+    wall = time.perf_counter() - w0
+    res = ThermalResult.chain([hot, cool])
+    kms = hot.meta["profile"]["kernel_ms"] + cool.meta["profile"]["kernel_ms"]
+    launches = hot.meta["profile"]["launches"] + cool.meta["profile"]["launches"]
+    alg = (48 * hot.substeps * n_heat + 44 * cool.substeps * n_cool) * P
+    gbs = alg / (kms * 1e-3) / 1e9
+    if not (np.isfinite(res.temperature_max).all() and np.isfinite(res.dose_cem43).all()):
+        raise RuntimeError("thermal benchmark produced non-finite values")
+    return {
+        "grid": list(shape), "dt_s": 0.05, "steps": n_heat + n_cool, "substeps": hot.substeps,
+        "focal_intensity_w_cm2": workloads.FOCAL_INTENSITY_W_M2 / 1e4, "q_scale": scale,
+        "peak_temperature_c": round(float(res.temperature_max.max()), 3),
+        "peak_cem43_min": float(res.dose_cem43.max()),
+        "wall_s": round(wall, 3), "kernel_ms": round(kms, 3), "launches": launches,
+        "roofline": {"bound": "hbm", "kernel": "k_pennes", "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "alg_bytes_per_step": 48 * P},
+    }
+
+
 def serving_leg(n_req: int, rates, seed: int, log) -> dict:
     """Mixed-size queue (128^3, 192^3, 256^3, 128x128x256; 70 % linear / 30 % Westervelt
     h = 1-2; steered bowls; fresh random media) replayed at Poisson arrivals, served by
@@ -190,6 +239,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=5, help="timed CPU baseline steps (0: skip)")
     ap.add_argument("--cpu-workers", type=int, default=1)
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--thermal-steps", type=int, default=600,
+                    help="Pennes steps heated by the 256^3 phasor (1/3 heating, 2/3 cooling; 0: skip)")
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests of the mixed-size serving queue (0: skip the serving leg)")
     ap.add_argument("--serving-rates", default="32,2", help="Poisson arrival rates [req/s]")
@@ -293,6 +344,11 @@ def main() -> int:
                             "GB_s": round(step_bytes / (step_ms * 1e-3) / 1e9, 1)}
 
     log(f"gpu: {value:.3f} solves/s; passes {json.dumps(passes)}")
+    thermal = None
+    if args.thermal_steps > 0 and args.workload == "cfg1":
+        n_heat = max(1, args.thermal_steps // 3)
+        thermal = thermal_leg(sess, slot, plan, wl, rec_steps, n_heat, args.thermal_steps - n_heat, log)
+        log(f"thermal: {json.dumps(thermal)}")
     sess.close()
     serving = None
     if args.serving_requests > 0:
@@ -331,6 +387,7 @@ def main() -> int:
                        "p99": round(float(np.percentile(lat_ms, 99)), 2)},
         "roofline": roofline,
         "passes": passes,
+        "thermal": thermal,
         "serving": serving,
         "cpu_baseline": cpu,
     }

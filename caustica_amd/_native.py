@@ -56,6 +56,7 @@ EXPORTS = (
     "cst_thermal_run",
     "cst_thermal_status",
     "cst_thermal_read",
+    "cst_thermal_heat_from_member",
     "cst_thermal_set_profiling",
     "cst_thermal_kernel_time",
 )
@@ -158,6 +159,7 @@ def _bind(lib):
         "cst_thermal_run": ([v, i], i),
         "cst_thermal_status": ([v, C.POINTER(i)], i),
         "cst_thermal_read": ([v, p, p, p], i),
+        "cst_thermal_heat_from_member": ([v, v, i, i, p, p, p, d, C.c_float], i),
         "cst_thermal_set_profiling": ([v, i], i),
         "cst_thermal_kernel_time": ([v, C.POINTER(d), C.POINTER(i64)], i),
     }

@@ -20,6 +20,7 @@
 
 #include "../../include/caustica_hip.h"
 #include "common.h"
+#include "thermal_state.h"
 
 using namespace cst;
 
@@ -157,26 +158,6 @@ __global__ void k_f32_to_dose(const float* __restrict__ in, double* __restrict__
 }
 
 }  // namespace
-
-struct cst_thermal {
-  int device = 0;
-  int nx = 0, ny = 0, nz = 0;
-  hipStream_t stream = nullptr;
-  char* block = nullptr;
-  float* theta[2] = {nullptr, nullptr};
-  float* tmax = nullptr;
-  double* dose = nullptr;
-  float* k = nullptr;
-  float* perf = nullptr;
-  float* coeff = nullptr;
-  float* q = nullptr;
-  unsigned* bad = nullptr;
-  int cur = 0;
-  bool have_q = false;
-  cst_thermal_desc d{};
-  bool ready = false;
-  Profiler prof;
-};
 
 extern "C" {
 

@@ -14,6 +14,7 @@
 #include "common.h"
 #include "kspace_launch.h"
 #include "kspace_types.h"
+#include "thermal_state.h"
 
 using namespace cst;
 
@@ -42,6 +43,31 @@ __global__ void k_finalize(const float2* __restrict__ acc, float2* __restrict__ 
     comp[i] = make_float2((float)(((double)s.x - (double)c.x) * scale),
                           (float)(((double)s.y - (double)c.y) * scale));
   }
+}
+
+struct HeatJoin {
+  const float2* p[kMaxHarmonics];
+  double f[kMaxHarmonics];
+  int nt;
+  const float* w;
+  const float* perf;
+  float qoff;
+  double scale;
+  int64_t n;
+  float* q;
+};
+
+// q = float32(scale * w * sum_j f_j |P_j|^2) - perf * qoff (the subtraction in float32)
+__global__ void __launch_bounds__(256) k_heat_join(HeatJoin j) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= j.n) return;
+  double acc = 0.0;
+  for (int t = 0; t < j.nt; ++t) {
+    const float2 v = j.p[t][i];
+    acc += j.f[t] * ((double)v.x * (double)v.x + (double)v.y * (double)v.y);
+  }
+  const float q = (float)(j.scale * (double)j.w[i] * acc);
+  j.q[i] = __fsub_rn(q, __fmul_rn(j.perf[i], j.qoff));
 }
 
 }  // namespace
@@ -639,6 +665,46 @@ int cst_session_kernel_time(cst_session* s, int kind, double* total_ms, int64_t*
   if (int r = s->prof.resolve()) return r;
   *total_ms = s->prof.ms[kind];
   *launches = s->prof.count[kind];
+  return CST_OK;
+}
+
+int cst_thermal_heat_from_member(cst_thermal* t, cst_session* s, int slot, int n_terms,
+                                 const int* harmonic_index, const double* factor, const float* w,
+                                 double scale, float q_offset) {
+  if (!t || !harmonic_index || !factor || !w) return fail(CST_ERR_ARG, "null argument");
+  if (int r = check_slot(s, slot)) return r;
+  if (!t->ready) return fail(CST_ERR_STATE, "thermal session not set up");
+  if (t->device != s->device) return fail(CST_ERR_ARG, "thermal and acoustic sessions on different devices");
+  Member& m = s->members[slot];
+  if (n_terms < 1 || n_terms > m.nh) return fail(CST_ERR_ARG, "n_terms must be 1..n_harmonics");
+  const int* rc = m.dev.rec;
+  if (rc[1] - rc[0] != t->nx || rc[3] - rc[2] != t->ny || rc[5] - rc[4] != t->nz)
+    return fail(CST_ERR_ARG, "thermal grid does not match the member's record region");
+  HeatJoin j{};
+  for (int k = 0; k < n_terms; ++k) {
+    const int hi = harmonic_index[k];
+    if (hi < 0 || hi >= m.nh) return fail(CST_ERR_ARG, "harmonic index out of range");
+    if (!m.finalized[hi]) return fail(CST_ERR_STATE, "phasor not finalized: call cst_member_result first");
+    j.p[k] = m.dev.comp[hi];
+    j.f[k] = factor[k];
+  }
+  j.nt = n_terms;
+  j.n = m.rec_count;
+  j.perf = t->perf;
+  j.qoff = q_offset;
+  j.scale = scale;
+  j.q = t->q;
+  HIPCHK(hipSetDevice(s->device));
+  HIPCHK(hipStreamSynchronize(s->stream));  // the finalized phasors are complete
+  // w is staged in the thermal theta[1] buffer (free until the first thermal sub-step)
+  float* wd = t->theta[1 - t->cur];
+  HIPCHK(hipMemcpyAsync(wd, w, (size_t)j.n * 4, hipMemcpyHostToDevice, t->stream));
+  j.w = wd;
+  const unsigned blocks = (unsigned)((j.n + 255) / 256);
+  hipLaunchKernelGGL(k_heat_join, dim3(blocks), dim3(256), 0, t->stream, j);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(t->stream));
+  t->have_q = true;
   return CST_OK;
 }
 

@@ -10,6 +10,7 @@ The sum is formed in float64 and stored float32.
 
 from __future__ import annotations
 
+import dataclasses
 import warnings
 from dataclasses import dataclass, field
 from typing import Any
@@ -184,3 +185,75 @@ def _harmonic_alpha(requested, alpha_f0, available, y, explicit) -> dict:
             out[n] = arr
         return out
     return {n: alpha_f0 for n in requested}
+
+
+@dataclass(frozen=True, eq=False)
+class DeviceHeatingSource:
+    """Heat source formed on the device from an acoustic member still resident in HBM.
+
+    Same physics and validation as ``HeatingSource.from_phasors`` (Q = sum_n alpha_n
+    |P_n|^2 / (rho c), float64 sum, float32 store), but the phasors never leave the
+    device: ``PennesSolver.solve`` hands it to ``cst_thermal_heat_from_member``, which
+    writes Q straight into the thermal session. Only the map alpha_f0 / (rho c) (float32)
+    crosses the bus. |P|^2 is formed in float64 from the float32 phasor components.
+    """
+
+    session: Any
+    slot: int
+    w: np.ndarray
+    terms: tuple          # ((harmonic index in the member, absorption factor), ...)
+    region: tuple
+    dx: float
+    harmonics: tuple
+    alpha_model: str
+    scale: float = 1.0
+    meta: dict[str, Any] = field(default_factory=dict)
+
+    @classmethod
+    def from_member(cls, session, slot: int, plan, medium, dx: float, *, rec_steps: int,
+                    harmonics=None, alpha_power_law_y=None) -> DeviceHeatingSource:
+        """Finalize the member's phasors on the device and describe the join.
+
+        ``plan`` is the member's MemberPlan (record region and harmonics), ``medium`` the
+        acoustic Medium, ``rec_steps`` the number of recorded steps (the phasor scale is
+        2 / rec_steps, as for the host result)."""
+        available = tuple(int(h) for h in plan.harmonics)
+        requested = available if harmonics is None else tuple(dict.fromkeys(int(n) for n in harmonics))
+        missing = [n for n in requested if n not in available]
+        if missing:
+            raise ValueError(f"harmonics {missing} were asked for but are not in the result "
+                             f"(available: {list(available)}).")
+        if any(n < 1 for n in requested):
+            raise ValueError(f"harmonic numbers must be >= 1, got {requested}")
+        if not (np.isfinite(dx) and dx > 0):
+            raise ValueError(f"dx must be a positive finite spacing in meters, got {dx}")
+        region = tuple(plan.rec)
+        shape = tuple(s.stop - s.start for s in region)
+        maps = {k: np.asarray(getattr(medium, k)[region], np.float64) for k in ("alpha", "rho", "c")}
+        if (maps["alpha"] < 0).any():
+            raise ValueError("alpha must be >= 0 Np/m (absorption cannot be negative).")
+        if (maps["rho"] <= 0).any() or (maps["c"] <= 0).any():
+            raise ValueError("rho and c must be > 0 everywhere to convert |P|^2 into intensity.")
+        # the same absorption-model rules as the host path (scalar factors per harmonic)
+        factors = _harmonic_alpha(requested, np.ones((), np.float64), available, alpha_power_law_y, None)
+        model = (ALPHA_MODEL_POWER_LAW if alpha_power_law_y is not None and max(requested) > 1
+                 else ALPHA_MODEL_F0)
+        w = np.ascontiguousarray(maps["alpha"] / (maps["rho"] * maps["c"]), dtype=np.float32)
+        terms = []
+        for n in requested:
+            hi = available.index(n)
+            session.result(slot, hi, 2.0 / rec_steps, with_pmax=False, copy=False)
+            terms.append((hi, float(factors[n])))
+        return cls(session=session, slot=int(slot), w=w, terms=tuple(terms), region=region,
+                   dx=float(dx), harmonics=requested, alpha_model=model,
+                   meta={"alpha_model": model, "alpha_power_law_y": alpha_power_law_y,
+                         "intensity_model": "local plane wave: I = |P|^2 / (2 rho c)"})
+
+    @property
+    def shape(self):
+        return tuple(self.w.shape)
+
+    def scaled(self, factor: float) -> DeviceHeatingSource:
+        """Same distribution, Q multiplied by ``factor`` (applied in float64 on the device)."""
+        return dataclasses.replace(self, scale=self.scale * float(factor),
+                                   meta={**self.meta, "scaled_by": self.scale * float(factor)})

@@ -151,10 +151,15 @@ class KSpaceSession:
         N.check(self._lib.cst_member_peaks(self._h, int(slot), int(first), int(count), N.ptr(out)))
         return out
 
-    def result(self, slot: int, hi: int, scale: float, with_pmax: bool = True):
+    def result(self, slot: int, hi: int, scale: float, with_pmax: bool = True, copy: bool = True):
+        """Finalize harmonic ``hi`` (float64 reduction, once) and copy phasor / p_max out;
+        ``copy=False`` only finalizes (the phasor stays in HBM for a device-side consumer)."""
         plan = self._plans[slot]
         R = int(np.prod([plan.rec3[1] - plan.rec3[0], plan.rec3[3] - plan.rec3[2],
                          plan.rec3[5] - plan.rec3[4]]))
+        if not copy:
+            N.check(self._lib.cst_member_result(self._h, int(slot), int(hi), float(scale), None, None))
+            return None, None
         ph = np.zeros(R, np.complex64)
         pm = np.zeros(R, np.float32) if with_pmax else None
         N.check(self._lib.cst_member_result(self._h, int(slot), int(hi), float(scale), N.ptr(ph),

@@ -23,7 +23,7 @@ import numpy as np
 
 from caustica_amd import _native as N
 from caustica_amd.core.backend import get_backend
-from caustica_amd.sensors import HeatingSource
+from caustica_amd.sensors import DeviceHeatingSource, HeatingSource
 from caustica_amd.thermal.properties import ThermalMedium
 
 log = logging.getLogger("caustica")
@@ -126,7 +126,7 @@ class PennesSolver:
                  arterial_temperature_c: float = ARTERIAL_TEMPERATURE_C,
                  boundary: str = "insulated", boundary_temperature_c: float | None = None,
                  reference_temperature_c: float | None = None, on_unstable: str = "refuse",
-                 finite_check_every: int = 25):
+                 finite_check_every: int = 25, profile: bool = False):
         if boundary not in ("insulated", "dirichlet"):
             raise ValueError(f"boundary must be 'insulated' or 'dirichlet', got {boundary!r}")
         if on_unstable not in ("refuse", "substep"):
@@ -146,6 +146,8 @@ class PennesSolver:
                                         else float(reference_temperature_c))
         self.on_unstable = on_unstable
         self.finite_check_every = int(finite_check_every)
+        # record the stencil kernel's device time (HIP events) into result.meta
+        self.profile = bool(profile)
 
     # ------------------------------------------------------- host assembly
     def _faces(self, k, dx):
@@ -195,6 +197,14 @@ class PennesSolver:
         if q is None:
             return None, "none"
         shape = medium.shape
+        if isinstance(q, DeviceHeatingSource):
+            if abs(q.dx - medium.dx) > 1e-12 * max(q.dx, medium.dx):
+                raise ValueError(f"the heat source was built on dx = {q.dx * 1e3:.6g} mm but this "
+                                 f"thermal medium has dx = {medium.dx * 1e3:.6g} mm.")
+            if q.shape != tuple(shape):
+                raise ValueError(f"the heat source covers the record region {q.region} (shape "
+                                 f"{q.shape}) but the thermal medium is {shape}.")
+            return q, f"DeviceHeatingSource({q.alpha_model})"
         if isinstance(q, HeatingSource):
             if abs(q.dx - medium.dx) > 1e-12 * max(q.dx, medium.dx):
                 raise ValueError(f"the HeatingSource was built on dx = {q.dx * 1e3:.6g} mm but this "
@@ -243,10 +253,16 @@ class PennesSolver:
         dt_sub = dt / n_sub
         coeff = np.ascontiguousarray((self._inv_rho_c(medium) * np.float32(dt_sub)).astype(np.float32))
         perf = np.ascontiguousarray(self._perf(medium))
+        q_dev = q_arr if isinstance(q_arr, DeviceHeatingSource) else None
+        q_off = 0.0
         if float(self.reference_temperature_c - self.arterial_temperature_c) != 0.0 and medium.is_perfused:
             # perfusion sink -perf (theta + (t_ref - T_a)): fold the constant into Q
-            off = (perf * np.float32(self.reference_temperature_c - self.arterial_temperature_c)).astype(np.float32)
-            q_arr = (-off if q_arr is None else (q_arr - off)).astype(np.float32)
+            q_off = float(np.float32(self.reference_temperature_c - self.arterial_temperature_c))
+            if q_dev is None:
+                off = (perf * np.float32(q_off)).astype(np.float32)
+                q_arr = (-off if q_arr is None else (q_arr - off)).astype(np.float32)
+        if q_dev is not None:
+            q_arr = None  # formed on the device after setup
         d0 = None
         if dose:
             if dose0 is not None:
@@ -279,6 +295,14 @@ class PennesSolver:
         samples, times = [], []
         try:
             N.check(lib.cst_thermal_setup(h, C.byref(desc)))
+            if self.profile:
+                N.check(lib.cst_thermal_set_profiling(h, 1))
+            if q_dev is not None:
+                idx = np.array([t[0] for t in q_dev.terms], np.int32)
+                fac = np.array([t[1] for t in q_dev.terms], np.float64)
+                N.check(lib.cst_thermal_heat_from_member(
+                    h, q_dev.session.handle, q_dev.slot, len(idx), N.ptr(idx), N.ptr(fac),
+                    N.ptr(q_dev.w), float(q_dev.scale), float(q_off)))
 
             def read(theta=True, tmax=False, dz=False):
                 out = [np.empty(shape, np.float32) if f else None for f in (theta, tmax, dz)]
@@ -306,6 +330,11 @@ class PennesSolver:
             if record_every is not None and (not times or times[-1] != n_steps * dt):
                 record(n_steps)
             theta, tmax, dz = read(True, True, dose)
+            prof = None
+            if self.profile:
+                ms, cnt = C.c_double(), C.c_int64()
+                N.check(lib.cst_thermal_kernel_time(h, C.byref(ms), C.byref(cnt)))
+                prof = {"kernel_ms": ms.value, "launches": cnt.value}
         finally:
             lib.cst_thermal_destroy(h)
         return ThermalResult(
@@ -332,6 +361,7 @@ class PennesSolver:
                 "arterial_temperature_c": self.arterial_temperature_c,
                 "reference_temperature_c": self.reference_temperature_c,
                 "dose_accumulated": bool(dose),
+                **({"profile": prof} if prof is not None else {}),
             },
         )
 

@@ -211,3 +211,32 @@ def serving_queue(n: int, rate_hz: float, seed: int = 4, shapes=None):
         shape = None if shapes is None else shapes[int(rng.integers(len(shapes)))]
         out.append(serving_request(rid, rng, t, shape=shape))
     return out
+
+
+# Pennes tissue of the thermal benchmark: k 0.5 W/m/K, rho C 3.6e6 J/m^3/K, perfusion
+# w_b = 0.5 kg/m^3/s with blood c_b 3617 J/kg/K (perfusion_rate = w_b / rho_b, so that
+# the solver's perfusion_rate * rho_b * C_b equals 0.5 * 3617 W/m^3/K), T_art = T0 = 37 C.
+PENNES_K = 0.5
+PENNES_RHO = 1000.0
+PENNES_C = 3600.0
+PENNES_PERFUSION_RATE = 0.5 / 1050.0
+FOCAL_INTENSITY_W_M2 = 150.0e4  # 150 W/cm^2
+
+
+def pennes_tissue(shape, dx: float):
+    """Homogeneous perfused tissue of the thermal benchmark on ``shape``."""
+    from caustica_amd.thermal import ThermalMedium
+
+    one = np.ones(shape, np.float32)
+    return ThermalMedium(k=one * PENNES_K, rho=one * PENNES_RHO, specific_heat=one * PENNES_C,
+                         perfusion=one * np.float32(PENNES_PERFUSION_RATE), dx=dx)
+
+
+def focal_intensity_scale(phasor_at_focus: complex, rho: float, c: float,
+                          target_w_m2: float = FOCAL_INTENSITY_W_M2) -> float:
+    """Factor on Q that brings the focal intensity |P|^2 / (2 rho c) to ``target_w_m2``
+    (Q scales as |P|^2, i.e. linearly in intensity)."""
+    i_focus = abs(phasor_at_focus) ** 2 / (2.0 * rho * c)
+    if not i_focus > 0:
+        raise ValueError("zero field at the focus: nothing to scale")
+    return float(target_w_m2 / i_focus)

@@ -181,6 +181,19 @@ int cst_thermal_run(cst_thermal* t, int n_substeps);
 int cst_thermal_status(cst_thermal* t, int* nonfinite);
 /* Copy theta, theta_max (float32) and the dose (float64 sum rounded to float32); any may be NULL. */
 int cst_thermal_read(cst_thermal* t, float* theta, float* theta_max, float* dose);
+/* Acoustic -> thermal join on the device (replaces sensors.py:193-295 and the host round
+ * trip of the phasor and Q). For every voxel of member `slot`'s record region:
+ *   q = float32(scale * w * sum_j factor[j] |P_j|^2) - perf * q_offset      (float32 ops)
+ * with P_j the member's finalized phasor of harmonic index harmonic_index[j]
+ * (cst_member_result has run for it), |P_j|^2 and the sum in float64, w = alpha_f0 / (rho c)
+ * a host float32 map over the region, factor[j] the harmonic's absorption scale (n^y for a
+ * power law, 1 at f0) and q_offset = T_ref - T_a, the perfusion offset that
+ * thermal/pennes.py folds into Q when the reference temperature is not arterial (0
+ * otherwise). The thermal session must be set up on the record region's shape; it runs
+ * with this heat source from then on. Synchronizes both sessions' streams. */
+int cst_thermal_heat_from_member(cst_thermal* t, cst_session* s, int slot, int n_terms,
+                                 const int* harmonic_index, const double* factor, const float* w,
+                                 double scale, float q_offset);
 int cst_thermal_set_profiling(cst_thermal* t, int on);
 int cst_thermal_kernel_time(cst_thermal* t, double* total_ms, int64_t* launches);
 
