@@ -799,27 +799,48 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     z_spec_put<M, ZV>(nl, sr, s);
     __syncthreads();
     z_c2r<M, ZV>(ll, t, lvalid, s);
-    const int ry = m.rec[3] - m.rec[2], rz = m.rec[5] - m.rec[4];
+    // descriptor fields used after the first store, read once here: a global store may
+    // alias the descriptor table as far as the compiler knows, and would force re-reads
+    // with vector loads inside the element loop
+    int cv[6], rb[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      cv[q] = m.conv[q];
+      rb[q] = m.rec[q];
+    }
+    const int nh = m.nh;
+    const bool nonlin = m.beta2dt != nullptr;
+    gptr<float2> accp[kMaxHarmonics], compp[kMaxHarmonics];
+#pragma unroll
+    for (int h = 0; h < kMaxHarmonics; ++h) {
+      accp[h] = G(m.acc[h]);
+      compp[h] = G(m.comp[h]);
+    }
+    const gptr<float> pmaxp = G(m.pmax);
+    unsigned* const peaksp = m.peaks;
+    const int ry = rb[3] - rb[2], rz = rb[5] - rb[4];
     unsigned pk = 0u;
     // store of the finished element (l, mm) at chunk offset i, its convergence peak and,
     // on a record step, its phasor (Kahan) and p_max updates
     auto finish = [&](int i, int l, int mm, float2 pv) {
       const int x = s.lx[l], y = s.ly[l];
       p[i] = pv;
-      if (x >= m.conv[0] && x < m.conv[1] && y >= m.conv[2] && y < m.conv[3]) {
+      if (x >= cv[0] && x < cv[1] && y >= cv[2] && y < cv[3]) {
         const int z0 = 2 * mm;
-        if (z0 >= m.conv[4] && z0 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.x)));
-        if (z0 + 1 >= m.conv[4] && z0 + 1 < m.conv[5]) pk = max(pk, __float_as_uint(fabsf(pv.y)));
+        if (z0 >= cv[4] && z0 < cv[5]) pk = max(pk, __float_as_uint(fabsf(pv.x)));
+        if (z0 + 1 >= cv[4] && z0 + 1 < cv[5]) pk = max(pk, __float_as_uint(fabsf(pv.y)));
       }
-      if (sm.record && x >= m.rec[0] && x < m.rec[1] && y >= m.rec[2] && y < m.rec[3]) {
+      if (sm.record && x >= rb[0] && x < rb[1] && y >= rb[2] && y < rb[3]) {
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
           const int zz = 2 * mm + e;
           const float v = e ? pv.y : pv.x;
-          if (zz >= m.rec[4] && zz < m.rec[5]) {
-            const long ri = ((long)(x - m.rec[0]) * ry + (y - m.rec[2])) * rz + (zz - m.rec[4]);
-            for (int h = 0; h < m.nh; ++h) {
-              const gptr<float2> acc = G(m.acc[h]), comp = G(m.comp[h]);
+          if (zz >= rb[4] && zz < rb[5]) {
+            const long ri = ((long)(x - rb[0]) * ry + (y - rb[2])) * rz + (zz - rb[4]);
+#pragma unroll
+            for (int h = 0; h < kMaxHarmonics; ++h) {
+              if (h >= nh) break;
+              const gptr<float2> acc = accp[h], comp = compp[h];
               const float2 eh = sm.eh[h];
               const float2 sacc = acc[ri];
               float2 cp = comp[ri];
@@ -830,8 +851,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
               acc[ri] = make_float2(tx, ty);
               comp[ri] = cp;
             }
-            const gptr<float> pmax = G(m.pmax);
-            pmax[ri] = fmaxf(pmax[ri], fabsf(v));
+            pmaxp[ri] = fmaxf(pmaxp[ri], fabsf(v));
           }
         }
       }
@@ -845,7 +865,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
         const float2 dv = s.a[l * Z::LSA + mm];
         float2 pv = pp[j];
         float2 c = rc[j];
-        if (m.beta2dt) {
+        if (nonlin) {
           c.x += bt[j].x * pv.x;
           c.y += bt[j].y * pv.y;
         }
@@ -893,7 +913,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
       for (int off = 32; off > 0; off >>= 1) pk = max(pk, (unsigned)__shfl_xor((int)pk, off));
       if ((threadIdx.x & 63) == 0 && pk) {
         const int slot = (blockIdx.x * ((Z::THREADS + 63) / 64) + (threadIdx.x >> 6)) % kPeakSlots;
-        atomicMax(m.peaks + (long)sm.peak_idx * kPeakWords + slot * kPeakStride, pk);
+        atomicMax(peaksp + (long)sm.peak_idx * kPeakWords + slot * kPeakStride, pk);
       }
     }
   } else {
