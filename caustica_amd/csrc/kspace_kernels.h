@@ -265,23 +265,24 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
 // Per column (y, zb): kappa / N at x-bin kx is sinc(c_ref dt / 2 |k|) / N with
 // |k|^2 = kx^2 + ky^2 + kz^2; the y and z derivative factors are column constants.
 struct XCol {
-  gptr<const float> kx_full, kx_d;
+  const float2* kxt;  // LDS: (k_x, derivative k_x) per x bin
   float kyz2, h2, inv_n, kyd, kzd;
   __device__ __forceinline__ float kappa(int kxi) const {
-    const float kx = kx_full[kxi];
+    const float kx = kxt[kxi].x;
     return sinc_q(h2 * (kx * kx + kyz2)) * inv_n;
   }
+  __device__ __forceinline__ float kxd(int kxi) const { return kxt[kxi].y; }
 };
 
-__device__ __forceinline__ XCol x_col(const MemberDev& m, const Shape& s, int y, int zb) {
+__device__ __forceinline__ XCol x_col(const MemberDev& m, const Shape& s, int y, int zb,
+                                      const float2* kxt) {
   const gptr<const float> kt = G(m.ktab);
   const gptr<const float> ky_full = kt + 2 * s.nx;
   const gptr<const float> ky_d = ky_full + s.ny;
   const gptr<const float> kz_full = ky_d + s.ny;
   const gptr<const float> kz_d = kz_full + s.nzh;
   XCol c;
-  c.kx_full = kt;
-  c.kx_d = kt + s.nx;
+  c.kxt = kxt;
   const float kyf = ky_full[y], kzf = kz_full[zb];
   c.kyz2 = kyf * kyf + kzf * kzf;
   c.h2 = m.half_cdt * m.half_cdt;
@@ -290,6 +291,29 @@ __device__ __forceinline__ XCol x_col(const MemberDev& m, const Shape& s, int y,
   c.kzd = kz_d[zb];
   return c;
 }
+
+// the x wavenumber pair table (k_x, derivative k_x) staged like the twiddles
+template <int L, int NT>
+struct KxStage {
+  static constexpr int Q = (L + NT - 1) / NT;
+  float2 v[Q];
+  __device__ __forceinline__ void load(const MemberDev& m) {
+    const gptr<const float> kt = G(m.ktab);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int i0 = threadIdx.x + q * NT;
+      const int i = i0 < L ? i0 : L - 1;
+      v[q] = make_float2(kt[i], kt[L + i]);
+    }
+  }
+  __device__ __forceinline__ void put(float2* kxt) const {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int i = threadIdx.x + q * NT;
+      if (i < L) kxt[i] = v[q];
+    }
+  }
+};
 
 // gradient: Q = IFFT_x(i kappa / N FFT_x(sp)); sg[0] = IFFT_x(kx_d i kappa / N FFT_x(sp)),
 // sg[1] = ky_d Q, sg[2] = kz_d Q (ky_d, kz_d are constant along x: one inverse
@@ -303,9 +327,12 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   constexpr int TC = P::TC;
   __shared__ float2 tile[L * TC];
   __shared__ float2 tw[L];
+  __shared__ float2 kxt[L];
   TwStage<L, TC * T> tws;
   tws.load(twg);
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
+  KxStage<L, TC * T> kxs;
+  kxs.load(m);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
   const int nt = (a.s.nzh + TC - 1) / TC;
   const int y = blockIdx.x / nt;
@@ -318,13 +345,14 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const gptr<const float2> src = G(m.sp);
   in_load<L, T>(t, [&](int n) { return ld(src, base + n * C); }, V);  // see k_ypass
   tws.put(tw);
+  kxs.put(kxt);
   __syncthreads();
   TileAt<TC> at{tile, col};
   in_compute<L, T, -1>(t, V, at, tw);
   __syncthreads();
   float2 X[S::R2][S::A];
   in_finish<L, T, -1>(t, at, X);
-  const XCol c = x_col(m, a.s, y, zb);
+  const XCol c = x_col(m, a.s, y, zb, kxt);
   float2 Z[S::R2][S::A];
 #pragma unroll
   for (int r = 0; r < S::R2; ++r) {
@@ -334,7 +362,7 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
       const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
       const float kp = c.kappa(kxi);
       X[r][k1] = make_float2(-kp * X[r][k1].y, kp * X[r][k1].x);  // i kappa / N X
-      const float kd = c.kx_d[kxi];
+      const float kd = c.kxd(kxi);
       Z[r][k1] = make_float2(kd * X[r][k1].x, kd * X[r][k1].y);
     }
   }
@@ -385,9 +413,12 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   constexpr int TC = P::TC;
   __shared__ float2 tile[L * TC];
   __shared__ float2 tw[L];
+  __shared__ float2 kxt[L];
   TwStage<L, TC * T> tws;
   tws.load(twg);
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
+  KxStage<L, TC * T> kxs;
+  kxs.load(m);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
   const int nt = (a.s.nzh + TC - 1) / TC;
   const int y = blockIdx.x / nt;
@@ -399,9 +430,10 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   float2 V[S::R1][S::B], Wy[S::R1][S::B], Wz[S::R1][S::B];
   const gptr<const float2> ux = G(m.sg[0]), uy = G(m.sg[1]), uz = G(m.sg[2]);
   in_load<L, T>(t, [&](int n) { return ld(ux, base + n * C); }, V);  // see k_ypass
-  const XCol c = x_col(m, a.s, y, zb);
+  const XCol c = x_col(m, a.s, y, zb, kxt);
   TileAt<TC> at{tile, col};
   tws.put(tw);
+  kxs.put(kxt);
   __syncthreads();
   // issued after the barrier: the first transform then waits for U_x only, and these
   // loads overlap it
@@ -427,7 +459,7 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
 #pragma unroll
     for (int k1 = 0; k1 < S::A; ++k1) {
       const int kxi = (k2 < S::B) ? k2 + S::B * k1 : 0;
-      const float kd = c.kx_d[kxi];
+      const float kd = c.kxd(kxi);
       const float kp = c.kappa(kxi);
       const float2 sum = make_float2(kd * Xx[r][k1].x + X[r][k1].x, kd * Xx[r][k1].y + X[r][k1].y);
       X[r][k1] = make_float2(-kp * sum.y, kp * sum.x);
