@@ -260,13 +260,177 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   }
 }
 
+// The two 3-spectrum y passes carry the y and z derivative factors, which are constant
+// along x, so the x passes move two spectra instead of three:
+//   after xgrad (sg0 = A, the x-derivative spectrum; sg1 = Q = i kappa / N FFT_x(p-hat)):
+//     k_ygrad: sg0 <- IFFT_y(A), sg1 <- IFFT_y(ky_d Q), sg2 <- kz_d IFFT_y(Q)
+//   after zu (sg0..sg2 = U_x, U_y, U_z, real along y):
+//     k_ydiv:  sg0 <- FFT_y(U_x), sg1 <- ky_d FFT_y(U_y) + kz_d FFT_y(U_z)
+// (ky_d varies along the transformed axis and multiplies in the spectral domain; kz_d is
+// the column's constant.)
+template <int L, int NT>
+struct KyStage {
+  static constexpr int Q = (L + NT - 1) / NT;
+  float v[Q];
+  __device__ __forceinline__ void load(const MemberDev& m, const Shape& sh) {
+    const gptr<const float> kyd = G(m.ktab) + 2 * sh.nx + sh.ny;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int i = threadIdx.x + q * NT;
+      v[q] = kyd[i < L ? i : L - 1];
+    }
+  }
+  __device__ __forceinline__ void put(float* kyt) const {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int i = threadIdx.x + q * NT;
+      if (i < L) kyt[i] = v[q];
+    }
+  }
+};
+
+// store a transformed column X (bin k2 + B k1 held as X[r][k1]) times a scalar
+template <int L, int T>
+__device__ __forceinline__ void y_store(int t, bool valid, gptr<float2> dst, long base, long stride,
+                                        const float2 (&X)[Split<L, T>::R2][Split<L, T>::A], float f) {
+  using S = Split<L, T>;
+  if (!valid) return;
+#pragma unroll
+  for (int r = 0; r < S::R2; ++r) {
+    const int k2 = t + T * r;
+    if (k2 < S::B) {
+#pragma unroll
+      for (int k1 = 0; k1 < S::A; ++k1)
+        dst[base + (long)(k2 + S::B * k1) * stride] = make_float2(f * X[r][k1].x, f * X[r][k1].y);
+    }
+  }
+}
+
+template <int L>
+__global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
+    k_ygrad(StepArgs a, const float2* __restrict__ twg) {
+  using P = Plan<L>;
+  using S = Split<L, P::T>;
+  constexpr int TC = P::TC;
+  constexpr int T = P::T;
+  __shared__ float2 tile[L * TC];
+  __shared__ float2 tw[L];
+  __shared__ float kyt[L];
+  TwStage<L, TC * T> tws;
+  tws.load(twg);
+  const MemberDev& m = a.mem[a.ids[blockIdx.z]];
+  KyStage<L, TC * T> kys;
+  kys.load(m, a.s);
+  const int col = threadIdx.x % TC, t = threadIdx.x / TC;
+  const int nt = (a.s.nzh + TC - 1) / TC;
+  const int x = blockIdx.x / nt;
+  const int zb0 = (blockIdx.x - x * nt) * TC + col;
+  const bool valid = zb0 < a.s.nzh;
+  const int zb = valid ? zb0 : 0;
+  const long S_ = a.s.ms;
+  const long base = (long)x * a.s.ps + zb;
+  const gptr<float2> g0 = G(m.sg[0]), g1 = G(m.sg[1]), g2 = G(m.sg[2]);
+  float2 VA[S::R1][S::B], VQ[S::R1][S::B];
+  in_load<L, T>(t, [&](int n) { return ld(g0, base + n * S_); }, VA);  // see k_ypass
+  in_load<L, T>(t, [&](int n) { return ld(g1, base + n * S_); }, VQ);
+  const float kzd = G(m.ktab)[2 * a.s.nx + 2 * a.s.ny + a.s.nzh + zb];
+  tws.put(tw);
+  kys.put(kyt);
+  __syncthreads();
+  TileAt<TC> at{tile, col};
+  float2 X[S::R2][S::A];
+  in_compute<L, T, +1>(t, VA, at, tw);
+  __syncthreads();
+  in_finish<L, T, +1>(t, at, X);
+  y_store<L, T>(t, valid, g0, base, S_, X, 1.f);
+  // ky_d Q: input bin n = n1 + A n2 of V[r][n2] (n1 = t + T r)
+#pragma unroll
+  for (int r = 0; r < S::R1; ++r) {
+    const int n1 = cmin(t + T * r, S::A - 1);
+#pragma unroll
+    for (int n2 = 0; n2 < S::B; ++n2) {
+      const float f = kyt[n1 + S::A * n2];
+      VA[r][n2] = make_float2(f * VQ[r][n2].x, f * VQ[r][n2].y);
+    }
+  }
+  __syncthreads();
+  in_compute<L, T, +1>(t, VA, at, tw);
+  __syncthreads();
+  in_finish<L, T, +1>(t, at, X);
+  y_store<L, T>(t, valid, g1, base, S_, X, 1.f);
+  __syncthreads();
+  in_compute<L, T, +1>(t, VQ, at, tw);
+  __syncthreads();
+  in_finish<L, T, +1>(t, at, X);
+  y_store<L, T>(t, valid, g2, base, S_, X, kzd);
+}
+
+template <int L>
+__global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
+    k_ydiv(StepArgs a, const float2* __restrict__ twg) {
+  using P = Plan<L>;
+  using S = Split<L, P::T>;
+  constexpr int TC = P::TC;
+  constexpr int T = P::T;
+  __shared__ float2 tile[L * TC];
+  __shared__ float2 tw[L];
+  __shared__ float kyt[L];
+  TwStage<L, TC * T> tws;
+  tws.load(twg);
+  const MemberDev& m = a.mem[a.ids[blockIdx.z]];
+  KyStage<L, TC * T> kys;
+  kys.load(m, a.s);
+  const int col = threadIdx.x % TC, t = threadIdx.x / TC;
+  const int nt = (a.s.nzh + TC - 1) / TC;
+  const int x = blockIdx.x / nt;
+  const int zb0 = (blockIdx.x - x * nt) * TC + col;
+  const bool valid = zb0 < a.s.nzh;
+  const int zb = valid ? zb0 : 0;
+  const long S_ = a.s.ms;
+  const long base = (long)x * a.s.ps + zb;
+  const gptr<float2> g0 = G(m.sg[0]), g1 = G(m.sg[1]), g2 = G(m.sg[2]);
+  float2 VX[S::R1][S::B], VY[S::R1][S::B], VZ[S::R1][S::B];
+  in_load<L, T>(t, [&](int n) { return ld(g0, base + n * S_); }, VX);  // see k_ypass
+  in_load<L, T>(t, [&](int n) { return ld(g1, base + n * S_); }, VY);
+  in_load<L, T>(t, [&](int n) { return ld(g2, base + n * S_); }, VZ);
+  const float kzd = G(m.ktab)[2 * a.s.nx + 2 * a.s.ny + a.s.nzh + zb];
+  tws.put(tw);
+  kys.put(kyt);
+  __syncthreads();
+  TileAt<TC> at{tile, col};
+  float2 X[S::R2][S::A], XY[S::R2][S::A];
+  in_compute<L, T, -1>(t, VX, at, tw);
+  __syncthreads();
+  in_finish<L, T, -1>(t, at, X);
+  y_store<L, T>(t, valid, g0, base, S_, X, 1.f);
+  __syncthreads();
+  in_compute<L, T, -1>(t, VY, at, tw);
+  __syncthreads();
+  in_finish<L, T, -1>(t, at, XY);
+  __syncthreads();
+  in_compute<L, T, -1>(t, VZ, at, tw);
+  __syncthreads();
+  in_finish<L, T, -1>(t, at, X);
+  // W = ky_d(k) FFT_y(U_y) + kz_d FFT_y(U_z), output bin k = k2 + B k1
+#pragma unroll
+  for (int r = 0; r < S::R2; ++r) {
+    const int k2 = cmin(t + T * r, S::B - 1);
+#pragma unroll
+    for (int k1 = 0; k1 < S::A; ++k1) {
+      const float f = kyt[k2 + S::B * k1];
+      X[r][k1] = make_float2(f * XY[r][k1].x + kzd * X[r][k1].x, f * XY[r][k1].y + kzd * X[r][k1].y);
+    }
+  }
+  y_store<L, T>(t, valid, g1, base, S_, X, 1.f);
+}
+
 // ================================================================ x passes
 // Block = (y, column tile of TC bins); element (kx, y, zb) at kx * ps + y * ms + zb.
 // Per column (y, zb): kappa / N at x-bin kx is sinc(c_ref dt / 2 |k|) / N with
-// |k|^2 = kx^2 + ky^2 + kz^2; the y and z derivative factors are column constants.
+// |k|^2 = kx^2 + ky^2 + kz^2.
 struct XCol {
   const float2* kxt;  // LDS: (k_x, derivative k_x) per x bin
-  float kyz2, h2, inv_n, kyd, kzd;
+  float kyz2, h2, inv_n;
   __device__ __forceinline__ float kappa(int kxi) const {
     const float kx = kxt[kxi].x;
     return sinc_q(h2 * (kx * kx + kyz2)) * inv_n;
@@ -278,17 +442,13 @@ __device__ __forceinline__ XCol x_col(const MemberDev& m, const Shape& s, int y,
                                       const float2* kxt) {
   const gptr<const float> kt = G(m.ktab);
   const gptr<const float> ky_full = kt + 2 * s.nx;
-  const gptr<const float> ky_d = ky_full + s.ny;
-  const gptr<const float> kz_full = ky_d + s.ny;
-  const gptr<const float> kz_d = kz_full + s.nzh;
+  const gptr<const float> kz_full = ky_full + 2 * s.ny;
   XCol c;
   c.kxt = kxt;
   const float kyf = ky_full[y], kzf = kz_full[zb];
   c.kyz2 = kyf * kyf + kzf * kzf;
   c.h2 = m.half_cdt * m.half_cdt;
   c.inv_n = m.inv_n;
-  c.kyd = ky_d[y];
-  c.kzd = kz_d[zb];
   return c;
 }
 
@@ -315,9 +475,8 @@ struct KxStage {
   }
 };
 
-// gradient: Q = IFFT_x(i kappa / N FFT_x(sp)); sg[0] = IFFT_x(kx_d i kappa / N FFT_x(sp)),
-// sg[1] = ky_d Q, sg[2] = kz_d Q (ky_d, kz_d are constant along x: one inverse
-// transform serves both).
+// gradient: sg[0] = IFFT_x(kx_d i kappa / N FFT_x(sp)), sg[1] = Q = IFFT_x(i kappa / N
+// FFT_x(sp)); the y and z factors (constant along x) are applied by k_ygrad.
 template <int L>
 __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     k_xgrad(StepArgs a, const float2* __restrict__ twg) {
@@ -384,17 +543,13 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
           }
         }
       } else {
-        const gptr<float2> dy = G(m.sg[1]), dz = G(m.sg[2]);
+        const gptr<float2> dq = G(m.sg[1]);  // Q; k_ygrad applies ky_d and kz_d
 #pragma unroll
         for (int r = 0; r < S::R1; ++r) {
           const int m2 = t + T * r;
           if (m2 < S::A) {
 #pragma unroll
-            for (int j = 0; j < S::B; ++j) {
-              const long o = base + (long)(m2 + S::A * j) * C;
-              dy[o] = make_float2(c.kyd * V[r][j].x, c.kyd * V[r][j].y);
-              dz[o] = make_float2(c.kzd * V[r][j].x, c.kzd * V[r][j].y);
-            }
+            for (int j = 0; j < S::B; ++j) dq[base + (long)(m2 + S::A * j) * C] = V[r][j];
           }
         }
       }
@@ -402,8 +557,8 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   }
 }
 
-// divergence: sp = IFFT_x( i kappa / N (kx_d FFT_x(sg[0]) + FFT_x(ky_d sg[1] + kz_d sg[2])) )
-// (the y and z factors are constant along x, so their two spectra share one transform)
+// divergence: sp = IFFT_x( i kappa / N (kx_d FFT_x(sg[0]) + FFT_x(sg[1])) ), with sg[1] the
+// y/z divergence terms already combined by k_ydiv
 template <int L>
 __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     k_xdiv(StepArgs a, const float2* __restrict__ twg) {
@@ -427,29 +582,23 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const int zb = valid ? zb0 : 0;
   const long C = a.s.ps;
   const long base = (long)y * a.s.ms + zb;
-  float2 V[S::R1][S::B], Wy[S::R1][S::B], Wz[S::R1][S::B];
-  const gptr<const float2> ux = G(m.sg[0]), uy = G(m.sg[1]), uz = G(m.sg[2]);
+  float2 V[S::R1][S::B], W[S::R1][S::B];
+  const gptr<const float2> ux = G(m.sg[0]), uw = G(m.sg[1]);
   in_load<L, T>(t, [&](int n) { return ld(ux, base + n * C); }, V);  // see k_ypass
   const XCol c = x_col(m, a.s, y, zb, kxt);
   TileAt<TC> at{tile, col};
   tws.put(tw);
   kxs.put(kxt);
   __syncthreads();
-  // issued after the barrier: the first transform then waits for U_x only, and these
-  // loads overlap it
-  in_load<L, T>(t, [&](int n) { return ld(uy, base + n * C); }, Wy);
-  in_load<L, T>(t, [&](int n) { return ld(uz, base + n * C); }, Wz);
+  // issued after the barrier: the first transform then waits for U_x only, and this
+  // load overlaps it
+  in_load<L, T>(t, [&](int n) { return ld(uw, base + n * C); }, W);
   in_compute<L, T, -1>(t, V, at, tw);
   __syncthreads();
   float2 Xx[S::R2][S::A];
   in_finish<L, T, -1>(t, at, Xx);
-#pragma unroll
-  for (int r = 0; r < S::R1; ++r)
-#pragma unroll
-    for (int i = 0; i < S::B; ++i)
-      V[r][i] = make_float2(c.kyd * Wy[r][i].x + c.kzd * Wz[r][i].x, c.kyd * Wy[r][i].y + c.kzd * Wz[r][i].y);
   __syncthreads();
-  in_compute<L, T, -1>(t, V, at, tw);
+  in_compute<L, T, -1>(t, W, at, tw);
   __syncthreads();
   float2 X[S::R2][S::A];
   in_finish<L, T, -1>(t, at, X);

@@ -24,4 +24,30 @@ bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, hi
   }
 }
 
+#define CST_Y3(NAME, KERNEL)                                                          \
+  bool NAME(int L, const StepArgs& a, const float2* tw, hipStream_t st) {           \
+    switch (L) {                                                                    \
+      CST_LENGTHS(CST_Y3_CASE)                                                      \
+      default:                                                                      \
+        return false;                                                               \
+    }                                                                               \
+  }
+
+#define CST_Y3_CASE(N)                                                              \
+  case N: {                                                                         \
+    using P = Plan<N>;                                                              \
+    dim3 g(a.s.nx * ((a.s.nzh + P::TC - 1) / P::TC), 1, a.nmem);                     \
+    hipLaunchKernelGGL((CST_Y3_KERNEL<N>), g, dim3(P::TC * P::T), 0, st, a, tw);     \
+    return true;                                                                    \
+  }
+
+#define CST_Y3_KERNEL k_ygrad
+CST_Y3(launch_ygrad, k_ygrad)
+#undef CST_Y3_KERNEL
+#define CST_Y3_KERNEL k_ydiv
+CST_Y3(launch_ydiv, k_ydiv)
+#undef CST_Y3_KERNEL
+#undef CST_Y3_CASE
+#undef CST_Y3
+
 }  // namespace cst

@@ -133,9 +133,10 @@ int launch_step(cst_session* s, StepArgs& a) {
   hipStream_t st = s->stream;
   bool ok = true;
   ok = ok && s->prof.run(0, st, [&] { return launch_xgrad(sh.nx, a, s->tw_x, st); });
-  if (sh.ny > 1) ok = ok && s->prof.run(1, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 1, st); });
+  // the 3-spectrum y passes also apply the y/z derivative factors: they run for ny == 1 too
+  ok = ok && s->prof.run(1, st, [&] { return launch_ygrad(sh.ny, a, s->tw_y, st); });
   ok = ok && s->prof.run(2, st, [&] { return launch_zu(M, s->variant[2], a, s->tw_z, s->rt, st); });
-  if (sh.ny > 1) ok = ok && s->prof.run(1, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 1, st); });
+  ok = ok && s->prof.run(1, st, [&] { return launch_ydiv(sh.ny, a, s->tw_y, st); });
   ok = ok && s->prof.run(3, st, [&] { return launch_xdiv(sh.nx, a, s->tw_x, st); });
   if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 0, st); });
   ok = ok && s->prof.run(5, st, [&] { return launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 1, st); });
