@@ -332,11 +332,13 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const gptr<float2> g0 = G(m.sg[0]), g1 = G(m.sg[1]), g2 = G(m.sg[2]);
   float2 VA[S::R1][S::B], VQ[S::R1][S::B];
   in_load<L, T>(t, [&](int n) { return ld(g0, base + n * S_); }, VA);  // see k_ypass
-  in_load<L, T>(t, [&](int n) { return ld(g1, base + n * S_); }, VQ);
   const float kzd = G(m.ktab)[2 * a.s.nx + 2 * a.s.ny + a.s.nzh + zb];
   tws.put(tw);
   kys.put(kyt);
   __syncthreads();
+  // issued after the barrier (the tables' wait covers every earlier load): the first
+  // transform overlaps this load
+  in_load<L, T>(t, [&](int n) { return ld(g1, base + n * S_); }, VQ);
   TileAt<TC> at{tile, col};
   float2 X[S::R2][S::A];
   in_compute<L, T, +1>(t, VA, at, tw);
@@ -391,12 +393,13 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const gptr<float2> g0 = G(m.sg[0]), g1 = G(m.sg[1]), g2 = G(m.sg[2]);
   float2 VX[S::R1][S::B], VY[S::R1][S::B], VZ[S::R1][S::B];
   in_load<L, T>(t, [&](int n) { return ld(g0, base + n * S_); }, VX);  // see k_ypass
-  in_load<L, T>(t, [&](int n) { return ld(g1, base + n * S_); }, VY);
-  in_load<L, T>(t, [&](int n) { return ld(g2, base + n * S_); }, VZ);
   const float kzd = G(m.ktab)[2 * a.s.nx + 2 * a.s.ny + a.s.nzh + zb];
   tws.put(tw);
   kys.put(kyt);
   __syncthreads();
+  // issued after the barrier (see k_ygrad): the first transform overlaps these loads
+  in_load<L, T>(t, [&](int n) { return ld(g1, base + n * S_); }, VY);
+  in_load<L, T>(t, [&](int n) { return ld(g2, base + n * S_); }, VZ);
   TileAt<TC> at{tile, col};
   float2 X[S::R2][S::A], XY[S::R2][S::A];
   in_compute<L, T, -1>(t, VX, at, tw);
@@ -475,10 +478,15 @@ struct KxStage {
   }
 };
 
+// Waves per SIMD requested for k_xgrad: four (128 VGPRs) where the compiler fits the
+// kernel without spilling, else no constraint.
+constexpr int xgrad_waves(int L) { return (L <= 256 && L != 240) ? 4 : 1; }
+
 // gradient: sg[0] = IFFT_x(kx_d i kappa / N FFT_x(sp)), sg[1] = Q = IFFT_x(i kappa / N
 // FFT_x(sp)); the y and z factors (constant along x) are applied by k_ygrad.
 template <int L>
 __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
+    __attribute__((amdgpu_waves_per_eu(xgrad_waves(L))))
     k_xgrad(StepArgs a, const float2* __restrict__ twg) {
   using P = Plan<L>;
   constexpr int T = P::T;
