@@ -11,8 +11,8 @@
 //
 // Blocking: a block owns a TY x TZ tile of the (y, z) plane and marches along x over a
 // chunk of XCH planes; theta and k of the planes x-1, x, x+1 live in registers, the
-// y/z neighbours of plane x come from an LDS tile with a one-cell halo. Each volume is
-// read once per step (halo re-reads hit L2).
+// y/z neighbours of plane x come from a double-buffered LDS tile with a one-cell halo.
+// Each volume is read once per step (halo re-reads hit L2).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -56,9 +56,26 @@ __device__ __forceinline__ float cem43_rate(float t) {
   return t > 43.f ? exp2f(d) : exp2f(2.f * d);
 }
 
+// Software-pipelined march along x: while plane x is computed, the loads for plane x+1
+// (its halo and the per-voxel coefficients) and for plane x+2 (centre theta and k) are in
+// flight. Two register sets alternate between "current" and "next" (the loop body is
+// unrolled twice), so no copy ever waits for a load still in flight. All loads are
+// unconditional with indices clamped into the grid (values of the clamped lanes are
+// never used), so the wait counts stay exact; the LDS plane is double buffered, one
+// barrier per plane.
+struct PennesPre {
+  float tp, kp;            // centre theta, k of plane x+1 (for plane x)
+  float hty, hky, htz, hkz;  // halo theta, k of plane x
+  float q, pf, cf, tx;     // heat source, perfusion, dt/(rho C), theta_max of plane x
+  double dz;               // dose of plane x
+};
+
+// FULL: nx is a multiple of XCH, every block marches exactly XCH planes and the march is
+// unrolled completely (no loop header for the wait-count pass to merge at).
+template <bool HAS_Q, bool DOSE, bool FULL>
 __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
-  __shared__ float sth[TY + 2][TZ + 2];
-  __shared__ float sk[TY + 2][TZ + 2];
+  __shared__ float sth[2][TY + 2][TZ + 2];
+  __shared__ float sk[2][TY + 2][TZ + 2];
   const int tz = threadIdx.x % TZ, ty = threadIdx.x / TZ;
   const int z = blockIdx.x * TZ + tz;
   const int y = blockIdx.y * TY + ty;
@@ -66,49 +83,60 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
   const int x1 = min(x0 + XCH, a.nx);
   const bool valid = (z < a.nz) && (y < a.ny);
   const long plane = (long)a.ny * a.nz;
-  const long col = valid ? (long)y * a.nz + z : 0;
-  auto th = [&](int x) { return a.th_in[(long)x * plane + col]; };
-  auto kk = [&](int x) { return a.k[(long)x * plane + col]; };
-  float tm = 0.f, km = 1.f, tc = 0.f, kc = 1.f;
-  if (valid) {
-    tc = th(x0);
-    kc = kk(x0);
-    if (x0 > 0) {
-      tm = th(x0 - 1);
-      km = kk(x0 - 1);
-    }
-  }
+  const int zc = min(z, a.nz - 1), yc = min(y, a.ny - 1);
+  const long col = (long)yc * a.nz + zc;
+  // halo roles: the first / last row of the tile loads the row above / below, the first /
+  // last column the cell before / after (clamped at the grid faces: unused there)
+  const bool hy = ty == 0 || ty == TY - 1;
+  const bool hz = tz == 0 || tz == TZ - 1;
+  const int yh = ty == 0 ? max(yc - 1, 0) : min(yc + 1, a.ny - 1);
+  const int zh = tz == 0 ? max(zc - 1, 0) : min(zc + 1, a.nz - 1);
+  const long colhy = (long)(hy ? yh : yc) * a.nz + zc;
+  const long colhz = (long)yc * a.nz + (hz ? zh : zc);
+  const int sy_h = ty == 0 ? 0 : TY + 1;
+  const int sz_h = tz == 0 ? 0 : TZ + 1;
+  const int xlast = a.nx - 1;
+  auto th = [&](int x, long c) { return a.th_in[(long)x * plane + c]; };
+  auto kk = [&](int x, long c) { return a.k[(long)x * plane + c]; };
+  // loads for plane x: its halo and coefficients, and the centre of plane x+1
+  auto prefetch = [&](int x, PennesPre& p) {
+    const int xc = min(x, xlast), xp = min(x + 1, xlast);
+    const long i = (long)xc * plane + col;
+    p.tp = th(xp, col);
+    p.kp = kk(xp, col);
+    p.hty = th(xc, colhy);
+    p.hky = kk(xc, colhy);
+    p.htz = th(xc, colhz);
+    p.hkz = kk(xc, colhz);
+    p.q = HAS_Q ? a.q[i] : 0.f;
+    p.pf = a.perf[i];
+    p.cf = a.coeff[i];
+    p.tx = a.tmax[i];
+    p.dz = DOSE ? a.dose[i] : 0.0;
+  };
+  const int xm = max(x0 - 1, 0);
+  float tm = th(xm, col), km = kk(xm, col);
+  float tc = th(x0, col), kc = kk(x0, col);
+  PennesPre P0, P1;
+  prefetch(x0, P0);
   const float iw = a.inv_dx2;
   bool bad = false;
-  for (int x = x0; x < x1; ++x) {
-    float tp = 0.f, kp = 1.f;
-    if (valid && x + 1 < a.nx) {
-      tp = th(x + 1);
-      kp = kk(x + 1);
+  auto step = [&](int x, int b, const PennesPre& c, PennesPre& n) {
+    prefetch(x + 1, n);
+    sth[b][ty + 1][tz + 1] = tc;
+    sk[b][ty + 1][tz + 1] = kc;
+    if (hy) {
+      sth[b][sy_h][tz + 1] = c.hty;
+      sk[b][sy_h][tz + 1] = c.hky;
+    }
+    if (hz) {
+      sth[b][ty + 1][sz_h] = c.htz;
+      sk[b][ty + 1][sz_h] = c.hkz;
     }
     __syncthreads();
-    sth[ty + 1][tz + 1] = tc;
-    sk[ty + 1][tz + 1] = kc;
-    const long pb = (long)x * plane;
-    if (ty == 0 && y > 0 && y - 1 < a.ny && z < a.nz) {
-      sth[0][tz + 1] = a.th_in[pb + (long)(y - 1) * a.nz + z];
-      sk[0][tz + 1] = a.k[pb + (long)(y - 1) * a.nz + z];
-    }
-    if (ty == TY - 1 && y + 1 < a.ny && z < a.nz) {
-      sth[TY + 1][tz + 1] = a.th_in[pb + (long)(y + 1) * a.nz + z];
-      sk[TY + 1][tz + 1] = a.k[pb + (long)(y + 1) * a.nz + z];
-    }
-    if (tz == 0 && z > 0 && y < a.ny) {
-      sth[ty + 1][0] = a.th_in[pb + (long)y * a.nz + z - 1];
-      sk[ty + 1][0] = a.k[pb + (long)y * a.nz + z - 1];
-    }
-    if (tz == TZ - 1 && z + 1 < a.nz && y < a.ny) {
-      sth[ty + 1][TZ + 1] = a.th_in[pb + (long)y * a.nz + z + 1];
-      sk[ty + 1][TZ + 1] = a.k[pb + (long)y * a.nz + z + 1];
-    }
-    __syncthreads();
+    const float tp = c.tp, kp = c.kp;
     if (valid) {
-      const long i = pb + col;
+      const long i = (long)x * plane + col;
       float work = 0.f;
       // x faces
       if (x > 0) work += hmean(km, kc) * iw * (tm - tc);
@@ -116,25 +144,25 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
       if (x + 1 < a.nx) work += hmean(kc, kp) * iw * (tp - tc);
       else if (a.dirichlet & 1) work += 2.f * kc * iw * (a.theta_wall - tc);
       // y faces
-      if (y > 0) work += hmean(sk[ty][tz + 1], kc) * iw * (sth[ty][tz + 1] - tc);
+      if (y > 0) work += hmean(sk[b][ty][tz + 1], kc) * iw * (sth[b][ty][tz + 1] - tc);
       else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
-      if (y + 1 < a.ny) work += hmean(kc, sk[ty + 2][tz + 1]) * iw * (sth[ty + 2][tz + 1] - tc);
+      if (y + 1 < a.ny) work += hmean(kc, sk[b][ty + 2][tz + 1]) * iw * (sth[b][ty + 2][tz + 1] - tc);
       else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
       // z faces
-      if (z > 0) work += hmean(sk[ty + 1][tz], kc) * iw * (sth[ty + 1][tz] - tc);
+      if (z > 0) work += hmean(sk[b][ty + 1][tz], kc) * iw * (sth[b][ty + 1][tz] - tc);
       else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
-      if (z + 1 < a.nz) work += hmean(kc, sk[ty + 1][tz + 2]) * iw * (sth[ty + 1][tz + 2] - tc);
+      if (z + 1 < a.nz) work += hmean(kc, sk[b][ty + 1][tz + 2]) * iw * (sth[b][ty + 1][tz + 2] - tc);
       else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
-      if (a.q) work += a.q[i];
-      work -= a.perf[i] * tc;
-      const float tn = tc + work * a.coeff[i];
+      if (HAS_Q) work += c.q;
+      work -= c.pf * tc;
+      const float tn = tc + work * c.cf;
       a.th_out[i] = tn;
-      a.tmax[i] = fmaxf(a.tmax[i], tn);
+      a.tmax[i] = fmaxf(c.tx, tn);
       bad |= !isfinite(tn);
-      if (a.do_dose) {
+      if (DOSE) {
         const float r_new = cem43_rate(tn + a.t_ref);
         const float r_old = cem43_rate(tc + a.t_ref);
-        const double d = a.dose[i] + ((double)r_new + (double)r_old) * a.half_dt_min;
+        const double d = c.dz + ((double)r_new + (double)r_old) * a.half_dt_min;
         a.dose[i] = d;
         bad |= !isfinite(d);
       }
@@ -143,6 +171,19 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
     km = kc;
     tc = tp;
     kc = kp;
+  };
+  if constexpr (FULL) {
+    static_assert(XCH % 2 == 0, "the unrolled march alternates two register sets");
+#pragma unroll
+    for (int j = 0; j < XCH; j += 2) {
+      step(x0 + j, 0, P0, P1);
+      step(x0 + j + 1, 1, P1, P0);
+    }
+  } else {
+    for (int x = x0; x < x1; x += 2) {
+      step(x, 0, P0, P1);
+      if (x + 1 < x1) step(x + 1, 1, P1, P0);
+    }
   }
   if (bad) atomicOr(a.bad, 1u);
 }
@@ -264,7 +305,20 @@ int cst_thermal_run(cst_thermal* t, int n_substeps) {
     a.th_in = t->theta[t->cur];
     a.th_out = t->theta[1 - t->cur];
     t->prof.run(0, t->stream, [&] {
-      hipLaunchKernelGGL(k_pennes, grid, dim3(TZ * TY), 0, t->stream, a);
+      const bool hq = a.q != nullptr, dd = a.do_dose != 0, full = a.nx % XCH == 0;
+      const dim3 bl(TZ * TY);
+      hipStream_t st = t->stream;
+      if (full) {
+        if (hq && dd) hipLaunchKernelGGL((k_pennes<true, true, true>), grid, bl, 0, st, a);
+        else if (hq) hipLaunchKernelGGL((k_pennes<true, false, true>), grid, bl, 0, st, a);
+        else if (dd) hipLaunchKernelGGL((k_pennes<false, true, true>), grid, bl, 0, st, a);
+        else hipLaunchKernelGGL((k_pennes<false, false, true>), grid, bl, 0, st, a);
+      } else {
+        if (hq && dd) hipLaunchKernelGGL((k_pennes<true, true, false>), grid, bl, 0, st, a);
+        else if (hq) hipLaunchKernelGGL((k_pennes<true, false, false>), grid, bl, 0, st, a);
+        else if (dd) hipLaunchKernelGGL((k_pennes<false, true, false>), grid, bl, 0, st, a);
+        else hipLaunchKernelGGL((k_pennes<false, false, false>), grid, bl, 0, st, a);
+      }
       return true;
     });
     HIPCHK(hipGetLastError());
