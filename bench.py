@@ -115,6 +115,78 @@ def cpu_baseline(wl, n_steps: int, workers: int) -> dict:
     }
 
 
+def profile_passes(sess, one_solve, plan, total_steps) -> tuple[dict, dict]:
+    """Per-pass event timing over one profiled solve: (passes, step summary)."""
+    sess.set_profiling(True)
+    one_solve()
+    sess.sync()
+    kt = sess.kernel_times()
+    sess.set_profiling(False)
+    nb = pass_bytes(plan.padded3, plan.beta2_dt is not None)
+    passes = {}
+    for name, (ms, cnt) in kt.items():
+        if cnt == 0 or name not in nb:
+            continue
+        avg = ms / cnt
+        gbs = nb[name] / (avg * 1e-3) / 1e9
+        passes[name] = {"launches": cnt, "avg_us": round(avg * 1e3, 2),
+                        "bytes_per_launch": nb[name], "GB_s": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    step_ms = sum(passes[k]["avg_us"] * passes[k]["launches"] for k in passes) / 1e3 / total_steps
+    moved = sum(nb[k] * passes[k]["launches"] for k in passes) / total_steps
+    ref_bytes = reference_step_bytes(plan.padded3, plan.beta2_dt is not None)
+    ref_gbs = ref_bytes / (step_ms * 1e-3) / 1e9
+    step = {"alg_bytes": ref_bytes, "moved_bytes": round(moved), "kernel_ms": round(step_ms, 4),
+            "GB_s": round(ref_gbs, 1), "frac": round(ref_gbs / HBM_PEAK_GBS, 4)}
+    return passes, step
+
+
+def westervelt_leg(log) -> dict:
+    """The 256 x 256 x 384 Westervelt solve (dx 0.25 mm, 20-cell sponge, layered medium
+    with B/A, 1.0 MPa bowl, CFL 0.2, 80 periods, harmonics 1-3 from the last 2): one
+    untimed and one timed solve with the member resident, then a profiled one."""
+    import numpy as np
+
+    from caustica_amd import _native as N
+    from caustica_amd import workloads
+    from caustica_amd.solvers.kspace.native import KSpaceSession
+    from caustica_amd.solvers.kspace.operators import plan_member
+    from caustica_amd.tune import autotune
+
+    log("westervelt: building the 256x256x384 workload")
+    wl = workloads.cfg2_westervelt()
+    plan = plan_member(wl.grid, wl.medium, wl.source, wl.spec, harmonics=wl.harmonics,
+                       reference_point=wl.reference_point, nonlinear=True)
+    spp = plan.spp
+    rec = wl.spec.n_record_periods * spp
+    total = wl.periods_total * spp
+    with KSpaceSession(plan.padded3, 2) as sess:
+        choice = autotune(sess, plan, use_cache=False)
+        slot = sess.add(plan, max_periods=wl.periods_total + 4)
+
+        def one_solve():
+            sess.reset(slot)
+            sess.schedule(slot, total - rec)
+            sess.advance([slot], total)
+            for hi in range(len(plan.harmonics)):
+                N.check(N.lib().cst_member_result(sess.handle, slot, hi, 2.0 / rec, None, None))
+
+        one_solve()
+        sess.sync()
+        t0 = time.perf_counter()
+        one_solve()
+        sess.sync()
+        dt = time.perf_counter() - t0
+        peaks = sess.peaks(slot, 0, wl.periods_total - wl.spec.n_record_periods)
+        if not np.isfinite(peaks).all():
+            raise RuntimeError("Westervelt benchmark produced non-finite values")
+        passes, step = profile_passes(sess, one_solve, plan, total)
+    return {"workload": wl.name, "grid": list(wl.grid.shape), "spp": spp,
+            "steps_per_solve": total, "harmonics": list(wl.harmonics), "variants": choice,
+            "solves_per_s": round(1.0 / dt, 4), "ms_per_solve": round(dt * 1e3, 1),
+            "step": step, "passes": passes}
+
+
 def thermal_leg(sess, slot, plan, wl, rec_steps: int, n_heat: int, n_cool: int, log) -> dict:
     """Pennes + CEM43 on the 256^3 grid, heated by the resident 256^3 phasor: the heat
     source is formed on the device (alpha |P|^2 / (rho c)), scaled to 150 W/cm^2 focal
@@ -255,6 +327,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=5, help="timed CPU baseline steps (0: skip)")
     ap.add_argument("--cpu-workers", type=int, default=1)
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-westervelt", dest="westervelt", action="store_false",
+                    help="skip the 256x256x384 Westervelt leg")
     ap.add_argument("--thermal-steps", type=int, default=600,
                     help="Pennes steps heated by the 256^3 phasor (1/3 heating, 2/3 cooling; 0: skip)")
     ap.add_argument("--serving-requests", type=int, default=24,
@@ -329,20 +403,7 @@ def main() -> int:
     passes = {}
     roofline = None
     if not args.no_profile:
-        sess.set_profiling(True)
-        one_solve()
-        device_sync(sess)
-        kt = sess.kernel_times()
-        sess.set_profiling(False)
-        nb = pass_bytes(plan.padded3, plan.beta2_dt is not None)
-        for name, (ms, cnt) in kt.items():
-            if cnt == 0 or name not in nb:
-                continue
-            avg = ms / cnt
-            gbs = nb[name] / (avg * 1e-3) / 1e9
-            passes[name] = {"launches": cnt, "avg_us": round(avg * 1e3, 2),
-                            "bytes_per_launch": nb[name], "GB_s": round(gbs, 1),
-                            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        passes, step = profile_passes(sess, one_solve, plan, total_steps)
         dom = max(passes, key=lambda k: passes[k]["avg_us"] * passes[k]["launches"])
         d = passes[dom]
         roofline = {
@@ -353,16 +414,14 @@ def main() -> int:
             "unit": "GB/s",
             "frac": d["frac"],
             "traffic": read_traffic(dom, wl.name),
+            "step": step,
         }
-        step_ms = sum(passes[k]["avg_us"] * passes[k]["launches"] for k in passes) / 1e3 / total_steps
-        moved = sum(nb[k] * passes[k]["launches"] for k in passes) / total_steps
-        ref_bytes = reference_step_bytes(plan.padded3, plan.beta2_dt is not None)
-        ref_gbs = ref_bytes / (step_ms * 1e-3) / 1e9
-        roofline["step"] = {"alg_bytes": ref_bytes, "moved_bytes": round(moved),
-                            "kernel_ms": round(step_ms, 4), "GB_s": round(ref_gbs, 1),
-                            "frac": round(ref_gbs / HBM_PEAK_GBS, 4)}
 
     log(f"gpu: {value:.3f} solves/s; passes {json.dumps(passes)}")
+    westervelt = None
+    if args.westervelt and args.workload == "cfg1":
+        westervelt = westervelt_leg(log)
+        log(f"westervelt: {json.dumps(westervelt)}")
     thermal = None
     if args.thermal_steps > 0 and args.workload == "cfg1":
         n_heat = max(1, args.thermal_steps // 3)
@@ -406,6 +465,7 @@ def main() -> int:
                        "p99": round(float(np.percentile(lat_ms, 99)), 2)},
         "roofline": roofline,
         "passes": passes,
+        "westervelt": westervelt,
         "thermal": thermal,
         "serving": serving,
         "cpu_baseline": cpu,
