@@ -17,6 +17,9 @@ Fixtures (all float32/complex64 as the upstream path produces them):
   cw_*.npz         whole CW solves (phasors, p_max, schedule metadata)
   pennes_*.npz     Pennes + CEM43 solves (final T, T_max, dose)
   heat_*.npz       HeatingSource.from_phasors Q maps
+  array_*.npz      spiral array geometry, delay-and-sum phases and voxelized sources
+
+``--only arrays`` writes the array fixtures alone.
 """
 
 from __future__ import annotations
@@ -321,5 +324,59 @@ def main(src_dir: str) -> None:
     _save("heat_power_law", p1=p1, p2=p2, alpha=alpha, rho=rho, c=cc, y=np.float64(1.1), q=hs.q)
 
 
+def arrays_fixtures(src_dir: str) -> None:
+    """Spiral geometry (production 128-element and a compact 32-element array), DAS
+    phases towards steered targets, and voxelized sources (indices, owning element,
+    per-voxel phase) on the grids the array tests use."""
+    if src_dir not in sys.path:
+        sys.path.insert(0, src_dir)
+    from caustica import Grid, PMLSpec
+    from caustica.arrays import archimedean_spiral
+
+    f0, c0 = 1.0e6, 1500.0
+    prod_arr = archimedean_spiral()
+    small = archimedean_spiral(n_elements=32, d_outer=0.030, d_inner=0.010, roc=0.030,
+                               active_fraction=0.60)
+    odd = archimedean_spiral(n_elements=7, d_outer=0.024, d_inner=0.006, roc=0.020,
+                             active_fraction=0.3, n_arc_samples=257)
+    targets = np.array([[0.0, 0.0, 0.100], [3e-3, -2e-3, 0.096], [-7e-3, 5e-3, 0.112]])
+    das = np.stack([prod_arr.das_phases(t, f0=f0, c0=c0) for t in targets])
+    small_target = np.array([3e-3, -2e-3, small.focal_length - 4e-3])
+    small_das = small.das_phases(small_target, f0=f0, c0=c0)
+
+    g_small = Grid(shape=(96, 96, 96), dx=0.5e-3, pml=PMLSpec(thickness=5e-3))
+    v0 = small.voxelize(g_small, apex_vox=(48, 48, 12), f0=f0, amplitude=1e5)
+    v1 = small.voxelize(g_small, apex_vox=(47, 50, 10), f0=f0, amplitude=2e5, phases=small_das)
+    dxp = 0.375e-3
+    g_prod = Grid(shape=(288, 288, 64), dx=dxp, pml=PMLSpec(thickness=6 * dxp))
+    v2 = prod_arr.voxelize(g_prod, apex_vox=(144, 144, 8), f0=f0, amplitude=1e5, phases=das[1])
+    _save("array_spiral",
+          prod_positions=prod_arr.positions, prod_normals=prod_arr.normals,
+          prod_elem_radius=np.float64(prod_arr.elem_radius),
+          small_positions=small.positions, small_normals=small.normals,
+          small_elem_radius=np.float64(small.elem_radius),
+          odd_positions=odd.positions, odd_normals=odd.normals,
+          odd_elem_radius=np.float64(odd.elem_radius),
+          targets=targets, prod_das=das, small_target=small_target, small_das=small_das)
+    _save("array_voxelize",
+          small_idx=v0.source.indices.astype(np.int16), small_elem=v0.element_of_voxel,
+          small_phase=v0.source.phases,
+          steer_idx=v1.source.indices.astype(np.int16), steer_elem=v1.element_of_voxel,
+          steer_phase=v1.source.phases,
+          prod_idx=v2.source.indices.astype(np.int16), prod_elem=v2.element_of_voxel,
+          prod_phase=v2.source.phases,
+          meta_json=_meta({"small": {"shape": [96, 96, 96], "dx": 0.5e-3, "apex": [48, 48, 12],
+                                     "label": v0.source.label},
+                           "steer": {"apex": [47, 50, 10]},
+                           "prod": {"shape": [288, 288, 64], "dx": dxp, "apex": [144, 144, 8],
+                                    "label": v2.source.label}}))
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference/src")
+    args = [a for a in sys.argv[1:] if not a.startswith("--") and a != "arrays"]
+    src = args[0] if args else "/root/reference/src"
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "arrays":
+        arrays_fixtures(src)
+    else:
+        main(src)
+        arrays_fixtures(src)
