@@ -929,7 +929,9 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
 // plain R2C of the current p. After the C2R every element is finished in one pass:
 // update, the source voxels of its line (values computed during the prefetch and
 // parked in LDS), the store of p, the convergence peak and the record sums.
-template <int M, int ZV>
+// NL = false is launched when no member of the batch is nonlinear: it holds no
+// registers for the beta map, which lets one more wave per SIMD in.
+template <int M, int ZV, bool NL>
 __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     k_zp(StepArgs a, const float2* __restrict__ twg, const float2* __restrict__ rtg, int update) {
   using Z = ZPlan<M, ZV>;
@@ -954,7 +956,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     // before the C2R), then the volumes (after it: their latency overlaps the transform)
     ZStage<M, ZV> stg;
     z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
-    float2 sr[CN::SPT], rc[CN::EPT], ab[CN::EPT], bt[CN::EPT];
+    float2 sr[CN::SPT], rc[CN::EPT], ab[CN::EPT], bt[NL ? CN::EPT : 1];
     z_spec_load<M, ZV>(G(m.sp), line0, nl, a.s, sr);
     const gptr<const int> sptr = G(m.src_ptr);
     const int sbeg = sptr[line0], send = sptr[line0 + nl];
@@ -974,7 +976,9 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     z_vol_load<M, ZV>(p, nl, pp);
     z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.rhoc2dt)) + e0, nl, rc);
     z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.absorb)) + e0, nl, ab);
-    if (m.beta2dt) z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.beta2dt)) + e0, nl, bt);
+    if constexpr (NL) {
+      if (m.beta2dt) z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.beta2dt)) + e0, nl, bt);
+    }
     z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
     if (threadIdx.x <= nl) s.sptr[threadIdx.x] = sp_l;
 #pragma unroll
@@ -998,7 +1002,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
       rb[q] = m.rec[q];
     }
     const int nh = m.nh;
-    const bool nonlin = m.beta2dt != nullptr;
+    const bool nonlin = NL && m.beta2dt != nullptr;
     gptr<float2> accp[kMaxHarmonics], compp[kMaxHarmonics];
 #pragma unroll
     for (int h = 0; h < kMaxHarmonics; ++h) {
@@ -1054,9 +1058,11 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
         const float2 dv = s.a[l * Z::LSA + mm];
         float2 pv = pp[j];
         float2 c = rc[j];
-        if (nonlin) {
-          c.x += bt[j].x * pv.x;
-          c.y += bt[j].y * pv.y;
+        if constexpr (NL) {
+          if (nonlin) {
+            c.x += bt[j].x * pv.x;
+            c.y += bt[j].y * pv.y;
+          }
         }
         const float2 s2 = s.sz[mm];
         pv.x -= c.x * dv.x;

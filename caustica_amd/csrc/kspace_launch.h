@@ -24,7 +24,8 @@ bool launch_xdiv(int L, const StepArgs& a, const float2* tw, hipStream_t st);
 // variant: thread layout of the z passes (ZPlan<M, V>), chosen per session
 bool launch_zu(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt,
                hipStream_t st);
+// nonlinear: some member of the batch has a beta map (picks the kernel that loads it)
 bool launch_zp(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt, int update,
-               hipStream_t st);
+               bool nonlinear, hipStream_t st);
 
 }  // namespace cst

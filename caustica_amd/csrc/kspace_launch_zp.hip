@@ -5,20 +5,24 @@
 namespace cst {
 
 template <int N, int V>
-static void go_zp(const StepArgs& a, const float2* tw, const float2* rt, int update, hipStream_t st) {
+static void go_zp(const StepArgs& a, const float2* tw, const float2* rt, int update, bool nonlinear,
+                  hipStream_t st) {
   using Z = ZPlan<N, V>;
   const int nl = a.s.nx * a.s.ny;
   dim3 g((nl + Z::LPB - 1) / Z::LPB, 1, a.nmem);
-  hipLaunchKernelGGL((k_zp<N, V>), g, dim3(Z::THREADS), 0, st, a, tw, rt, update);
+  if (nonlinear)
+    hipLaunchKernelGGL((k_zp<N, V, true>), g, dim3(Z::THREADS), 0, st, a, tw, rt, update);
+  else
+    hipLaunchKernelGGL((k_zp<N, V, false>), g, dim3(Z::THREADS), 0, st, a, tw, rt, update);
 }
 
 bool launch_zp(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt, int update,
-               hipStream_t st) {
+               bool nonlinear, hipStream_t st) {
   switch (M) {
 #define CST_ZP(N)                                          \
   case N:                                                  \
-    if (variant == 1) go_zp<N, 1>(a, tw, rt, update, st);  \
-    else go_zp<N, 0>(a, tw, rt, update, st);               \
+    if (variant == 1) go_zp<N, 1>(a, tw, rt, update, nonlinear, st);  \
+    else go_zp<N, 0>(a, tw, rt, update, nonlinear, st);               \
     return true;
     CST_LENGTHS(CST_ZP)
 #undef CST_ZP

@@ -139,7 +139,11 @@ int launch_step(cst_session* s, StepArgs& a) {
   ok = ok && s->prof.run(1, st, [&] { return launch_ydiv(sh.ny, a, s->tw_y, st); });
   ok = ok && s->prof.run(3, st, [&] { return launch_xdiv(sh.nx, a, s->tw_x, st); });
   if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 0, st); });
-  ok = ok && s->prof.run(5, st, [&] { return launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 1, st); });
+  bool nonlinear = false;
+  for (int i = 0; i < a.nmem; ++i) nonlinear = nonlinear || s->members[a.ids[i]].dev.beta2dt != nullptr;
+  ok = ok && s->prof.run(5, st, [&] {
+    return launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 1, nonlinear, st);
+  });
   if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 0, st); });
   if (!ok) return fail(CST_ERR_UNSUPPORTED, "no kernel instantiated for this shape");
   HIPCHK(hipGetLastError());
@@ -155,7 +159,7 @@ int prime(cst_session* s, int slot) {
   a.ids[0] = slot;
   a.sm[0].peak_idx = -1;
   const int M = s->s.nz / 2;
-  if (!launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 0, s->stream))
+  if (!launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 0, false, s->stream))
     return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
   if (s->s.ny > 1 && !launch_y(s->s.ny, -1, a, s->tw_y, 0, s->stream))
     return fail(CST_ERR_UNSUPPORTED, "no kernel for ny");
