@@ -96,7 +96,10 @@ __device__ __forceinline__ void in_load(int t, Load load, float2 (&V)[Split<L, T
   }
 }
 
-template <int L, int T, int DIR, class At>
+// TT: the twiddle table is stored transposed, tw[k2 A + n1] = w^(n1 k2), for passes
+// whose consecutive lanes hold consecutive n1 of one line (the z passes): the natural
+// layout would have them read at stride k2 and collide on LDS banks.
+template <int L, int T, int DIR, bool TT = false, class At>
 __device__ __forceinline__ void in_compute(int t, float2 (&V)[Split<L, T>::R1][Split<L, T>::B], At at,
                                            const float2* tw) {
   using S = Split<L, T>;
@@ -106,7 +109,8 @@ __device__ __forceinline__ void in_compute(int t, float2 (&V)[Split<L, T>::R1][S
     if (n1 < S::A) {
       Dft<S::B, DIR>::run(V[r]);
 #pragma unroll
-      for (int k2 = 0; k2 < S::B; ++k2) *at(n1 + S::A * k2) = twmul<DIR>(V[r][k2], tw[n1 * k2]);
+      for (int k2 = 0; k2 < S::B; ++k2)
+        *at(n1 + S::A * k2) = twmul<DIR>(V[r][k2], tw[TT ? k2 * S::A + n1 : n1 * k2]);
     }
   }
 }
@@ -719,12 +723,13 @@ __device__ __forceinline__ void z_stage_load(const float2* twg, const float2* rt
   // every load is unconditional (indices clamped into range): the wait-count pass can
   // then wait for exactly the loads a consumer needs instead of for all of them
   constexpr int TH = ZStage<M, ZV>::THREADS;
+  constexpr int A = Split<M, ZPlan<M, ZV>::ZT>::A;
   const gptr<const float2> szg = G(reinterpret_cast<const float2*>(m.sponge));
 #pragma unroll
   for (int q = 0; q < ZStage<M, ZV>::TPT; ++q) {
     const int i = threadIdx.x + q * TH;
     const int im = i < M ? i : M - 1;
-    st.tw[q] = twg[im];
+    st.tw[q] = twg[(im % A) * (im / A)];  // transposed table (see in_compute)
     st.sz[q] = szg[im];
     st.rt[q] = rtg[i <= M ? i : M];
   }
@@ -818,7 +823,7 @@ __device__ __forceinline__ void z_c2r(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   float2 V[S::R1][S::B];
   in_load<M, Z::ZT>(t, [&](int n) { return c2r_pre<M>(la, n, s.rt); }, V);
   __syncthreads();  // a consumed
-  if (lvalid) in_compute<M, Z::ZT, +1>(t, V, LinePadAt<S::A>{le}, s.tw);
+  if (lvalid) in_compute<M, Z::ZT, +1, true>(t, V, LinePadAt<S::A>{le}, s.tw);
   __syncthreads();
   float2 X[S::R2][S::A];
   if (lvalid) {
@@ -845,7 +850,7 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   float2* le = s.e + ll * Z::LSE;
   float2 V[S::R1][S::B];
   in_load<M, Z::ZT>(t, [&](int n) { return la[n]; }, V);
-  if (lvalid) in_compute<M, Z::ZT, -1>(t, V, LinePadAt<S::A>{le}, s.tw);
+  if (lvalid) in_compute<M, Z::ZT, -1, true>(t, V, LinePadAt<S::A>{le}, s.tw);
   __syncthreads();  // e written, a consumed
   if (lvalid) {
     float2 X[S::R2][S::A];
