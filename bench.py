@@ -37,15 +37,18 @@ def pass_bytes(padded3, nonlinear: bool) -> dict:
     H = nx ny (nz/2 + 1) half-spectrum bins, P = nx ny nz voxels. The y and z derivative
     factors ride in the 3-spectrum y passes, so the x passes carry two spectra:
     xgrad reads p-hat and writes A and Q; the y pass after it reads A, Q and writes three
-    gradient spectra; the one after zu reads three and writes U_x-hat and the combined
-    y/z divergence terms; xdiv reads those two and writes the divergence spectrum."""
+    gradient spectra; zu reads each gradient spectrum and velocity-state spectrum and
+    writes the velocity state back (the velocity lives on the device as its z-spectrum);
+    the y pass after zu reads the three velocity spectra and writes U_x-hat and the
+    combined y/z divergence terms; xdiv reads those two and writes the divergence
+    spectrum."""
     nx, ny, nz = padded3
     P = nx * ny * nz
     H = nx * ny * (nz // 2 + 1)
     return {
         "xgrad": 8 * H + 16 * H,
         "y3": 16 * H + 24 * H,                     # ygrad and ydiv both move 40 H
-        "zu": 24 * H + 12 * P + 8 * P + 12 * P + 24 * H,  # spectra, u, dt/rho + absorb, u, u-hat
+        "zu": 24 * H + 24 * H + 8 * P + 24 * H,    # gradient spectra, u-hat, dt/rho + absorb, u-hat
         "xdiv": 16 * H + 8 * H,
         "y1": 2 * 8 * H,
         "zp": 8 * H + 4 * P + 8 * P + (4 * P if nonlinear else 0) + 4 * P + 8 * H,

@@ -47,6 +47,9 @@ EXPORTS = (
     "cst_member_result",
     "cst_member_get_state",
     "cst_member_set_state",
+    "cst_member_state_size",
+    "cst_member_get_raw",
+    "cst_member_set_raw",
     "cst_session_set_variant",
     "cst_session_set_profiling",
     "cst_session_kernel_time",
@@ -150,6 +153,9 @@ def _bind(lib):
         "cst_member_result": ([v, i, i, d, p, p], i),
         "cst_member_get_state": ([v, i, p, p, p, p, C.POINTER(i64)], i),
         "cst_member_set_state": ([v, i, i64, p, p, p, p], i),
+        "cst_member_state_size": ([v, C.POINTER(C.c_size_t)], i),
+        "cst_member_get_raw": ([v, i, p, C.c_size_t, C.POINTER(i64)], i),
+        "cst_member_set_raw": ([v, i, p, C.c_size_t, i64], i),
         "cst_session_set_variant": ([v, i, i], i),
         "cst_session_set_profiling": ([v, i], i),
         "cst_session_kernel_time": ([v, i, C.POINTER(d), C.POINTER(i64)], i),

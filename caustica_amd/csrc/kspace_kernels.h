@@ -268,7 +268,7 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
 // along x, so the x passes move two spectra instead of three:
 //   after xgrad (sg0 = A, the x-derivative spectrum; sg1 = Q = i kappa / N FFT_x(p-hat)):
 //     k_ygrad: sg0 <- IFFT_y(A), sg1 <- IFFT_y(ky_d Q), sg2 <- kz_d IFFT_y(Q)
-//   after zu (sg0..sg2 = U_x, U_y, U_z, real along y):
+//   after zu (su0..su2 = U_x, U_y, U_z, the velocity state, real along y):
 //     k_ydiv:  sg0 <- FFT_y(U_x), sg1 <- ky_d FFT_y(U_y) + kz_d FFT_y(U_z)
 // (ky_d varies along the transformed axis and multiplies in the spectral domain; kz_d is
 // the column's constant.)
@@ -394,16 +394,17 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const int zb = valid ? zb0 : 0;
   const long S_ = a.s.ms;
   const long base = (long)x * a.s.ps + zb;
-  const gptr<float2> g0 = G(m.sg[0]), g1 = G(m.sg[1]), g2 = G(m.sg[2]);
+  const gptr<float2> u0 = G(m.su[0]), u1 = G(m.su[1]), u2 = G(m.su[2]);
+  const gptr<float2> g0 = G(m.sg[0]), g1 = G(m.sg[1]);
   float2 VX[S::R1][S::B], VY[S::R1][S::B], VZ[S::R1][S::B];
-  in_load<L, T>(t, [&](int n) { return ld(g0, base + n * S_); }, VX);  // see k_ypass
+  in_load<L, T>(t, [&](int n) { return ld(u0, base + n * S_); }, VX);  // see k_ypass
   const float kzd = G(m.ktab)[2 * a.s.nx + 2 * a.s.ny + a.s.nzh + zb];
   tws.put(tw);
   kys.put(kyt);
   __syncthreads();
   // issued after the barrier (see k_ygrad): the first transform overlaps these loads
-  in_load<L, T>(t, [&](int n) { return ld(g1, base + n * S_); }, VY);
-  in_load<L, T>(t, [&](int n) { return ld(g2, base + n * S_); }, VZ);
+  in_load<L, T>(t, [&](int n) { return ld(u1, base + n * S_); }, VY);
+  in_load<L, T>(t, [&](int n) { return ld(u2, base + n * S_); }, VZ);
   TileAt<TC> at{tile, col};
   float2 X[S::R2][S::A], XY[S::R2][S::A];
   in_compute<L, T, -1>(t, VX, at, tw);
@@ -870,10 +871,16 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   }
 }
 
-// velocity update, one axis per block. Block b covers line group g and axis ax with
-// b = 24 * (g / 8) + 8 * ax + (g % 8): blocks are dealt to the 8 XCDs round-robin, so
-// the three axis blocks of a line group run back to back on one XCD and its L2 serves
-// the second and third reads of dt/rho and absorb.
+// velocity update, one axis per block. The velocity state is kept as its z-spectrum
+// (su), the form the divergence needs next, so a component is read and written once
+// per step as a spectrum instead of as a real volume plus its spectrum: per axis
+// 24 H bytes instead of 16 H + 8 P. The block C2Rs the gradient spectrum, keeps the
+// real gradient in registers, C2Rs the velocity spectrum, updates
+// u = ((u - dt/rho g) absorb) sponge in LDS and R2Cs it back into su.
+// Block b covers line group g and axis ax with b = 24 * (g / 8) + 8 * ax + (g % 8):
+// blocks are dealt to the 8 XCDs round-robin, so the three axis blocks of a line group
+// run back to back on one XCD and its L2 serves the second and third reads of dt/rho
+// and absorb.
 template <int M, int ZV>
 __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     k_zu(StepArgs a, const float2* __restrict__ twg, const float2* __restrict__ rtg) {
@@ -891,20 +898,33 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   const int nl = cmin(Z::LPB, nlines - (int)line0);
   const int ll = threadIdx.x / Z::ZT, t = threadIdx.x % Z::ZT;
   const bool lvalid = ll < nl;
-  const long e0 = line0 * M;  // first float2 of the block's chunk
-  const gptr<float2> spec = G(m.sg[ax]);
-  const gptr<float2> u = G(reinterpret_cast<float2*>(m.u[ax])) + e0;
-  // loads in the order they are consumed: staged tables, spectrum (before the C2R),
-  // then the volumes (after it: their latency overlaps the transform)
+  const long e0 = line0 * M;  // first float2 of the block's chunk of a real volume
+  const gptr<float2> gspec = G(m.sg[ax]);
+  const gptr<float2> uspec = G(m.su[ax]);
+  const float inz = m.inv_nz;
+  // loads in the order they are consumed: staged tables, gradient spectrum, velocity
+  // spectrum, then the maps
   ZStage<M, ZV> stg;
   z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
-  float2 sr[CN::SPT], uu[CN::EPT], dto[CN::EPT], ab[CN::EPT];
-  z_spec_load<M, ZV>(spec, line0, nl, a.s, sr);
-  z_vol_load<M, ZV>(u, nl, uu);
+  float2 sr[CN::SPT], ur[CN::SPT], dto[CN::EPT], ab[CN::EPT];
+  z_spec_load<M, ZV>(gspec, line0, nl, a.s, sr);
+  z_spec_load<M, ZV>(uspec, line0, nl, a.s, ur);
   z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.dto_rho)) + e0, nl, dto);
   z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.absorb)) + e0, nl, ab);
   z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
   z_spec_put<M, ZV>(nl, sr, s);
+  __syncthreads();
+  z_c2r<M, ZV>(ll, t, lvalid, s);
+  float2 gg[CN::EPT];
+#pragma unroll
+  for (int j = 0; j < CN::EPT; ++j) {
+    const int i = threadIdx.x + j * Z::THREADS;
+    const int ic = i < nl * M ? i : nl * M - 1;
+    const int l = ic / M, mm = ic - l * M;
+    gg[j] = s.a[l * Z::LSA + mm];
+  }
+  __syncthreads();
+  z_spec_put<M, ZV>(nl, ur, s);
   __syncthreads();
   z_c2r<M, ZV>(ll, t, lvalid, s);
 #pragma unroll
@@ -913,21 +933,75 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     if (i < nl * M) {
       const int l = i / M, mm = i - l * M;
       const float sxy = s.sxy[l];
-      const float2 gg = s.a[l * Z::LSA + mm];
       const float2 s2 = s.sz[mm];
-      float2 uv = uu[j];
-      uv.x -= dto[j].x * gg.x;
-      uv.y -= dto[j].y * gg.y;
+      float2 uv = s.a[l * Z::LSA + mm];
+      uv.x *= inz;
+      uv.y *= inz;
+      uv.x -= dto[j].x * gg[j].x;
+      uv.y -= dto[j].y * gg[j].y;
       uv.x *= ab[j].x;
       uv.y *= ab[j].y;
       uv.x *= sxy * s2.x;
       uv.y *= sxy * s2.y;
-      u[i] = uv;
       s.a[l * Z::LSA + mm] = uv;
     }
   }
   __syncthreads();
-  z_r2c<M, ZV>(ll, t, lvalid, s, spec + s.row[lvalid ? ll : 0]);
+  z_r2c<M, ZV>(ll, t, lvalid, s, uspec + s.row[lvalid ? ll : 0]);
+}
+
+// Conversion of one velocity component between its state form and a real volume, for
+// reading and writing the state: dir > 0: R2C of the real volume held in sg[ax]'s memory
+// into su[ax]; dir < 0: C2R of su[ax], divided by nz, into sg[ax]'s memory as a real
+// volume (z contiguous, the layout of p). sg is scratch between steps.
+template <int M, int ZV>
+__global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
+    k_zconv(StepArgs a, const float2* __restrict__ twg, const float2* __restrict__ rtg, int ax, int dir) {
+  using Z = ZPlan<M, ZV>;
+  using CN = ZCount<M, ZV>;
+  __shared__ ZLds<M, ZV> s;
+  const MemberDev& m = a.mem[a.ids[blockIdx.z]];
+  const int nlines = a.s.nx * a.s.ny;
+  const long line0 = (long)blockIdx.x * Z::LPB;
+  const int nl = cmin(Z::LPB, nlines - (int)line0);
+  const int ll = threadIdx.x / Z::ZT, t = threadIdx.x % Z::ZT;
+  const bool lvalid = ll < nl;
+  const gptr<float2> real = G(m.sg[ax]) + line0 * M;
+  const gptr<float2> uspec = G(m.su[ax]);
+  ZStage<M, ZV> stg;
+  z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
+  if (dir > 0) {
+    float2 vv[CN::EPT];
+    z_vol_load<M, ZV>(real, nl, vv);
+    z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
+#pragma unroll
+    for (int j = 0; j < CN::EPT; ++j) {
+      const int i = threadIdx.x + j * Z::THREADS;
+      if (i < nl * M) {
+        const int l = i / M, mm = i - l * M;
+        s.a[l * Z::LSA + mm] = vv[j];
+      }
+    }
+    __syncthreads();
+    z_r2c<M, ZV>(ll, t, lvalid, s, uspec + s.row[lvalid ? ll : 0]);
+  } else {
+    const float inz = m.inv_nz;
+    float2 sr[CN::SPT];
+    z_spec_load<M, ZV>(uspec, line0, nl, a.s, sr);
+    z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
+    z_spec_put<M, ZV>(nl, sr, s);
+    __syncthreads();
+    z_c2r<M, ZV>(ll, t, lvalid, s);
+#pragma unroll
+    for (int j = 0; j < CN::EPT; ++j) {
+      const int i = threadIdx.x + j * Z::THREADS;
+      if (i < nl * M) {
+        const int l = i / M, mm = i - l * M;
+        const float2 v = s.a[l * Z::LSA + mm];
+        real[i] = make_float2(v.x * inz, v.y * inz);
+      }
+    }
+  }
 }
 
 // pressure update (+ Westervelt), source, convergence peak, record; or (update == 0) a

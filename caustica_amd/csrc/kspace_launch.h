@@ -24,6 +24,9 @@ bool launch_xdiv(int L, const StepArgs& a, const float2* tw, hipStream_t st);
 // variant: thread layout of the z passes (ZPlan<M, V>), chosen per session
 bool launch_zu(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt,
                hipStream_t st);
+// velocity component ax between state and real volume (k_zconv): dir > 0 real -> state
+bool launch_zconv(int M, const StepArgs& a, const float2* tw, const float2* rt, int ax, int dir,
+                  hipStream_t st);
 // nonlinear: some member of the batch has a beta map (picks the kernel that loads it)
 bool launch_zp(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt, int update,
                bool nonlinear, hipStream_t st);

@@ -28,4 +28,26 @@ bool launch_zu(int M, int variant, const StepArgs& a, const float2* tw, const fl
   }
 }
 
+template <int N>
+static void go_zconv(const StepArgs& a, const float2* tw, const float2* rt, int ax, int dir, hipStream_t st) {
+  using Z = ZPlan<N, 0>;
+  const int nl = a.s.nx * a.s.ny;
+  dim3 g((nl + Z::LPB - 1) / Z::LPB, 1, a.nmem);
+  hipLaunchKernelGGL((k_zconv<N, 0>), g, dim3(Z::THREADS), 0, st, a, tw, rt, ax, dir);
+}
+
+bool launch_zconv(int M, const StepArgs& a, const float2* tw, const float2* rt, int ax, int dir,
+                  hipStream_t st) {
+  switch (M) {
+#define CST_ZC(N)                              \
+  case N:                                      \
+    go_zconv<N>(a, tw, rt, ax, dir, st);       \
+    return true;
+    CST_LENGTHS(CST_ZC)
+#undef CST_ZC
+    default:
+      return false;
+  }
+}
+
 }  // namespace cst

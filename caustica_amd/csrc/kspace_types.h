@@ -24,9 +24,11 @@ constexpr int kPeakWords = kPeakSlots * kPeakStride;  // per period
 
 struct MemberDev {
   float* p;
-  float* u[3];
   float2* sp;
   float2* sg[3];
+  // velocity state: the z-spectrum (unnormalized R2C along z, same row layout as the
+  // other spectra) of each component; the real u is C2R(su) / nz, formed in the z pass
+  float2* su[3];
   const float* dto_rho;
   const float* rhoc2dt;
   const float* absorb;
@@ -47,7 +49,7 @@ struct MemberDev {
   int nh;
   float half_cdt;        // c_ref * dt / 2
   float inv_n;           // 1 / (nx ny nz): the inverse-transform normalization
-  int pad_;
+  float inv_nz;          // 1 / nz: the z round trip of the velocity state
 };
 
 // Global-address-space view of a device pointer read from a table. Loads and stores

@@ -150,14 +150,19 @@ int launch_step(cst_session* s, StepArgs& a) {
   return CST_OK;
 }
 
-// R2C + forward y of the member's current p into sp (after set_state).
-int prime(cst_session* s, int slot) {
+StepArgs single_member(const cst_session* s, int slot) {
   StepArgs a{};
   a.mem = s->d_table;
   a.s = s->s;
   a.nmem = 1;
   a.ids[0] = slot;
   a.sm[0].peak_idx = -1;
+  return a;
+}
+
+// R2C + forward y of the member's current p into sp (after set_state).
+int prime(cst_session* s, int slot) {
+  StepArgs a = single_member(s, slot);
   const int M = s->s.nz / 2;
   if (!launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 0, false, s->stream))
     return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
@@ -329,8 +334,8 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   const int64_t nsp = sh.nz + sh.nx + sh.ny;
   const int maxp = std::max(d->max_periods, 1);
   Carver c;
-  const size_t o_state = c.take(4 * P * 4);
-  const size_t o_spec = c.take(4 * H * 8);
+  const size_t o_state = c.take(P * 4);      // p
+  const size_t o_spec = c.take(7 * H * 8);   // sp, sg0..sg2, su0..su2 (velocity state)
   const size_t o_maps = c.take(nmaps * P * 4);
   const size_t o_k = c.take(nk * 4);
   const size_t o_sp = c.take(nsp * 4);
@@ -356,12 +361,11 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   m.block_bytes = c.total;
   m.used = true;
   MemberDev& md = m.dev;
-  float* st = (float*)(base + o_state);
-  md.p = st;
-  for (int a = 0; a < 3; ++a) md.u[a] = st + (a + 1) * P;
+  md.p = (float*)(base + o_state);
   float2* spc = (float2*)(base + o_spec);
   md.sp = spc;
   for (int a = 0; a < 3; ++a) md.sg[a] = spc + (a + 1) * H;
+  for (int a = 0; a < 3; ++a) md.su[a] = spc + (a + 4) * H;
   float* maps = (float*)(base + o_maps);
   md.dto_rho = maps;
   md.rhoc2dt = maps + P;
@@ -386,6 +390,7 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   md.nh = d->n_harmonics;
   md.half_cdt = (float)(0.5 * d->c_ref * d->dt);
   md.inv_n = (float)(1.0 / (double)P);
+  md.inv_nz = (float)(1.0 / (double)sh.nz);
 
   m.dt = d->dt;
   m.f0 = d->f0;
@@ -400,8 +405,8 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stm);
   };
-  hipError_t e = hipMemsetAsync(base + o_state, 0, 4 * P * 4, stm);
-  if (e == hipSuccess) e = hipMemsetAsync(base + o_spec, 0, 4 * H * 8, stm);
+  hipError_t e = hipMemsetAsync(base + o_state, 0, P * 4, stm);
+  if (e == hipSuccess) e = hipMemsetAsync(base + o_spec, 0, 7 * H * 8, stm);
   if (e == hipSuccess) e = up((void*)md.dto_rho, d->dt_over_rho, P * 4);
   if (e == hipSuccess) e = up((void*)md.rhoc2dt, d->rhoc2_dt, P * 4);
   if (e == hipSuccess) e = up((void*)md.absorb, d->absorb, P * 4);
@@ -494,8 +499,8 @@ int cst_member_reset(cst_session* s, int slot) {
   const int64_t P = (int64_t)sh.nx * sh.ny * sh.nz;
   const int64_t H = (int64_t)sh.nx * sh.ps;
   HIPCHK(hipSetDevice(s->device));
-  HIPCHK(hipMemsetAsync(m.dev.p, 0, 4 * P * 4, s->stream));
-  HIPCHK(hipMemsetAsync(m.dev.sp, 0, 4 * H * 8, s->stream));
+  HIPCHK(hipMemsetAsync(m.dev.p, 0, P * 4, s->stream));
+  HIPCHK(hipMemsetAsync(m.dev.sp, 0, 7 * H * 8, s->stream));
   for (int h = 0; h < m.nh; ++h) {
     HIPCHK(hipMemsetAsync(m.dev.acc[h], 0, m.rec_count * 8, s->stream));
     HIPCHK(hipMemsetAsync(m.dev.comp[h], 0, m.rec_count * 8, s->stream));
@@ -623,10 +628,17 @@ int cst_member_get_state(cst_session* s, int slot, float* p, float* u0, float* u
   Member& m = s->members[slot];
   const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
   HIPCHK(hipSetDevice(s->device));
-  float* dst[4] = {p, u0, u1, u2};
-  const float* src[4] = {m.dev.p, m.dev.u[0], m.dev.u[1], m.dev.u[2]};
-  for (int i = 0; i < 4; ++i)
-    if (dst[i]) HIPCHK(hipMemcpyAsync(dst[i], src[i], P * 4, hipMemcpyDeviceToHost, s->stream));
+  if (p) HIPCHK(hipMemcpyAsync(p, m.dev.p, P * 4, hipMemcpyDeviceToHost, s->stream));
+  // velocity: C2R of the state spectrum into the (scratch) gradient spectrum's memory
+  float* u[3] = {u0, u1, u2};
+  StepArgs a = single_member(s, slot);
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!u[ax]) continue;
+    if (!launch_zconv(s->s.nz / 2, a, s->tw_z, s->rt, ax, -1, s->stream))
+      return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(u[ax], m.dev.sg[ax], P * 4, hipMemcpyDeviceToHost, s->stream));
+  }
   HIPCHK(hipStreamSynchronize(s->stream));
   if (n) *n = m.n;
   return CST_OK;
@@ -639,10 +651,58 @@ int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, co
   Member& m = s->members[slot];
   const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
   HIPCHK(hipSetDevice(s->device));
-  const float* src[4] = {p, u0, u1, u2};
-  float* dst[4] = {m.dev.p, m.dev.u[0], m.dev.u[1], m.dev.u[2]};
-  for (int i = 0; i < 4; ++i)
-    HIPCHK(hipMemcpyAsync(dst[i], src[i], P * 4, hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(m.dev.p, p, P * 4, hipMemcpyHostToDevice, s->stream));
+  const float* u[3] = {u0, u1, u2};
+  StepArgs a = single_member(s, slot);
+  for (int ax = 0; ax < 3; ++ax) {
+    HIPCHK(hipMemcpyAsync(m.dev.sg[ax], u[ax], P * 4, hipMemcpyHostToDevice, s->stream));
+    if (!launch_zconv(s->s.nz / 2, a, s->tw_z, s->rt, ax, +1, s->stream))
+      return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  m.n = n;
+  m.primed = false;
+  return CST_OK;
+}
+
+static size_t raw_state_bytes(const cst_session* s) {
+  const Shape& sh = s->s;
+  return (size_t)sh.nx * sh.ny * sh.nz * 4 + 3 * (size_t)sh.nx * sh.ps * 8;
+}
+
+int cst_member_state_size(cst_session* s, size_t* bytes) {
+  if (!s || !bytes) return fail(CST_ERR_ARG, "null argument");
+  *bytes = raw_state_bytes(s);
+  return CST_OK;
+}
+
+int cst_member_get_raw(cst_session* s, int slot, void* buf, size_t bytes, int64_t* n) {
+  if (int r = check_slot(s, slot)) return r;
+  if (!buf || bytes != raw_state_bytes(s)) return fail(CST_ERR_ARG, "buffer size differs from cst_member_state_size");
+  Member& m = s->members[slot];
+  const size_t pb = (size_t)s->s.nx * s->s.ny * s->s.nz * 4, hb = (size_t)s->s.nx * s->s.ps * 8;
+  HIPCHK(hipSetDevice(s->device));
+  char* dst = static_cast<char*>(buf);
+  HIPCHK(hipMemcpyAsync(dst, m.dev.p, pb, hipMemcpyDeviceToHost, s->stream));
+  for (int ax = 0; ax < 3; ++ax)
+    HIPCHK(hipMemcpyAsync(dst + pb + ax * hb, m.dev.su[ax], hb, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (n) *n = m.n;
+  return CST_OK;
+}
+
+int cst_member_set_raw(cst_session* s, int slot, const void* buf, size_t bytes, int64_t n) {
+  if (int r = check_slot(s, slot)) return r;
+  if (!buf || bytes != raw_state_bytes(s)) return fail(CST_ERR_ARG, "buffer size differs from cst_member_state_size");
+  if (n < 0) return fail(CST_ERR_ARG, "negative step");
+  Member& m = s->members[slot];
+  const size_t pb = (size_t)s->s.nx * s->s.ny * s->s.nz * 4, hb = (size_t)s->s.nx * s->s.ps * 8;
+  HIPCHK(hipSetDevice(s->device));
+  const char* src = static_cast<const char*>(buf);
+  HIPCHK(hipMemcpyAsync(m.dev.p, src, pb, hipMemcpyHostToDevice, s->stream));
+  for (int ax = 0; ax < 3; ++ax)
+    HIPCHK(hipMemcpyAsync(m.dev.su[ax], src + pb + ax * hb, hb, hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   m.n = n;
   m.primed = false;

@@ -7,8 +7,9 @@ device table gpu_db.json) so a plan-first runner keeps its gates. What differs o
 hip target (the MI355X row, or any row with a measured ``bw_eff``):
 
 * memory is the library's own carve of one member (``hip_memory``: one allocation in
-  256-byte aligned pieces -- state, four padded half spectra, coefficient maps, 1-D k and
-  sponge tables, source CSR, record sums, convergence peak slots), exact to the byte; the
+  256-byte aligned pieces -- p, seven padded half spectra (four work spectra and the
+  velocity state), coefficient maps, 1-D k and sponge tables, source CSR, record sums,
+  convergence peak slots), exact to the byte; the
   upstream inventory of the same solve is reported beside it
   (``vram_bytes_reference_model``);
 * time is a bandwidth model: each of the eight passes of a step is HBM-bound, so a step
@@ -141,8 +142,8 @@ def hip_inventory(padded3, *, nonlinear: bool, n_harmonics: int, rec_elems: int,
     H = nx * (ny * ms + 16)
     ns = max(int(n_src), 1)
     b = {
-        "state (p + u)": _al(4 * P * 4),
-        "half spectra (4, padded rows)": _al(4 * H * 8),
+        "pressure (p)": _al(P * 4),
+        "half spectra (sp, sg0-2, velocity state su0-2)": _al(7 * H * 8),
         "property maps": _al((4 if nonlinear else 3) * P * 4),
         "k tables": _al(2 * (nx + ny + nzh) * 4),
         "sponge profiles": _al((nz + nx + ny) * 4),
@@ -164,12 +165,12 @@ def hip_memory(plan: MemberPlan, max_periods: int | None = None) -> MemoryModel:
 
 # ------------------------------------------------------------------- time
 def step_bytes(padded3, nonlinear: bool, n_harmonics: int = 1, recording: bool = False) -> int:
-    """HBM bytes one hip step moves: 224 H + 48 P (+4 P nonlinear; +(32 n_h + 8) P on a
+    """HBM bytes one hip step moves: 248 H + 24 P (+4 P nonlinear; +(32 n_h + 8) P on a
     record step), H = nx ny (nz/2 + 1) spectrum bins, P = nx ny nz voxels."""
     nx, ny, nz = (int(v) for v in padded3)
     P = nx * ny * nz
     H = nx * ny * (nz // 2 + 1)
-    b = 224 * H + 48 * P + (4 * P if nonlinear else 0)
+    b = 248 * H + 24 * P + (4 * P if nonlinear else 0)
     if recording:
         b += (32 * n_harmonics + 8) * P
     return int(b)

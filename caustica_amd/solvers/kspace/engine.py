@@ -180,10 +180,13 @@ def _control_loop(sess: KSpaceSession, slot: int, plan: MemberPlan, solver_name,
         state = load_checkpoint(checkpoint.path, fingerprint)
         if state is not None:
             f = state["fields"]
-            u = [np.zeros(plan.padded3, np.float32) for _ in range(3)]
-            for i, c in enumerate(ucomp):
-                u[c] = f[f"u{i}"].reshape(plan.padded3)
-            sess.set_state(slot, state["n"], f["p"], *u)
+            if "device_state" in f:
+                sess.set_raw(slot, state["n"], f["device_state"])
+            else:
+                u = [np.zeros(plan.padded3, np.float32) for _ in range(3)]
+                for i, c in enumerate(ucomp):
+                    u[c] = f[f"u{i}"].reshape(plan.padded3)
+                sess.set_state(slot, state["n"], f["p"], *u)
             n = state["n"]
             periods_done = state["periods_done"]
             prev_peak = state["prev_peak"]
@@ -193,10 +196,12 @@ def _control_loop(sess: KSpaceSession, slot: int, plan: MemberPlan, solver_name,
             stage = state["stage"]
 
     def _write_ckpt(stage_now: str) -> None:
+        # the real p and u for readers, and the exact device state for a bitwise resume
         (p, *u), _n = sess.get_state(slot)
         fields = {"p": p.reshape(plan.padded)}
         for i, c in enumerate(ucomp):
             fields[f"u{i}"] = u[c].reshape(plan.padded)
+        fields["device_state"], _n = sess.get_raw(slot)
         write_checkpoint(checkpoint.path, stage=stage_now, n=n, periods_done=periods_done,
                          prev_peak=prev_peak, converged=converged,
                          converged_period=converged_period, history=history, fields=fields,
@@ -215,7 +220,7 @@ def _control_loop(sess: KSpaceSession, slot: int, plan: MemberPlan, solver_name,
     warned = [False]
 
     def _snapshot() -> np.ndarray:
-        (p, *_u), _n = sess.get_state(slot)
+        (p, *_u), _n = sess.get_state(slot, with_u=False)
         p = p.reshape(plan.padded)
         if nd == 1:
             return p[: grid.shape[0]].copy()

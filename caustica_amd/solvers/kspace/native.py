@@ -166,12 +166,29 @@ class KSpaceSession:
                                             N.ptr(pm)))
         return ph, pm
 
-    def get_state(self, slot: int):
+    def get_state(self, slot: int, with_u: bool = True):
+        """([p, u0, u1, u2], n) as real volumes of the padded shape (u None unless
+        ``with_u``); the velocity is formed from its device spectrum."""
         shp = self.padded3
-        arrs = [np.empty(shp, np.float32) for _ in range(4)]
+        arrs = [np.empty(shp, np.float32)] + [np.empty(shp, np.float32) if with_u else None
+                                              for _ in range(3)]
         n = C.c_int64()
         N.check(self._lib.cst_member_get_state(self._h, int(slot), *[N.ptr(a) for a in arrs], C.byref(n)))
         return arrs, int(n.value)
+
+    def get_raw(self, slot: int):
+        """(bytes of the exact device state, n): restoring it with ``set_raw`` resumes the
+        trajectory bit for bit."""
+        size = C.c_size_t()
+        N.check(self._lib.cst_member_state_size(self._h, C.byref(size)))
+        buf = np.empty(size.value, np.uint8)
+        n = C.c_int64()
+        N.check(self._lib.cst_member_get_raw(self._h, int(slot), N.ptr(buf), buf.nbytes, C.byref(n)))
+        return buf, int(n.value)
+
+    def set_raw(self, slot: int, n: int, buf) -> None:
+        b = np.ascontiguousarray(buf, np.uint8)
+        N.check(self._lib.cst_member_set_raw(self._h, int(slot), N.ptr(b), b.nbytes, C.c_int64(int(n))))
 
     def set_state(self, slot: int, n: int, p, u0, u1, u2) -> None:
         arrs = [np.ascontiguousarray(np.reshape(a, self.padded3), np.float32) for a in (p, u0, u1, u2)]

@@ -178,7 +178,7 @@ class MemberPlan:
         H = nx * (ny * ((nz // 2 + 1 + 15) // 16 * 16) + 16)  # padded spectrum rows/planes
         R = int(np.prod(self.rec_shape))
         nmaps = 3 + (self.beta2_dt is not None)
-        return int(16 * P + 32 * H + 4 * nmaps * P + R * (16 * len(self.harmonics) + 4) + (1 << 20))
+        return int(4 * P + 56 * H + 4 * nmaps * P + R * (16 * len(self.harmonics) + 4) + (1 << 20))
 
 
 def plan_member(grid, medium, source, spec, *, harmonics=(1,), record_region=None,

@@ -133,11 +133,21 @@ int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out)
  * (interleaved re, im) over the record box, = (Kahan sum - compensation) * scale
  * evaluated in float64 on the device; pmax (may be NULL) = float32 max |p|. Synchronizes. */
 int cst_member_result(cst_session* s, int slot, int hi, double scale, float* phasor, float* pmax);
-/* Checkpoint state (engine.py:344-371): p, u0, u1, u2 (padded shape) and step n. */
+/* Solver state as real volumes (the state engine.py:344-371 checkpoints): p, u0, u1, u2
+ * (padded shape, each may be NULL in get_state) and step n. The velocity is held on the
+ * device as its z-spectrum: get_state forms u by a C2R along z, set_state by an R2C, so a
+ * get/set round trip changes u by float32 transform rounding. Synchronizes. */
 int cst_member_get_state(cst_session* s, int slot, float* p, float* u0, float* u1, float* u2,
                          int64_t* n);
 int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, const float* u0,
                          const float* u1, const float* u2);
+/* The exact device state as bytes (p, then the three velocity spectra), for checkpoints
+ * that resume bit for bit (io/checkpoint.py:127-226 with the resume contract of
+ * tests/test_checkpoint.py:61-83). state_size gives the byte count for the session's
+ * shape; get_raw / set_raw refuse any other size. Synchronizes. */
+int cst_member_state_size(cst_session* s, size_t* bytes);
+int cst_member_get_raw(cst_session* s, int slot, void* buf, size_t bytes, int64_t* n);
+int cst_member_set_raw(cst_session* s, int slot, const void* buf, size_t bytes, int64_t n);
 
 /* Select the thread-layout variant (0 or 1) of a tunable pass (0 y, 1 x, 2 zu, 3 zp) for
  * every later step of the session; the autotuner times both and keeps the faster. */

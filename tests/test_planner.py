@@ -46,16 +46,16 @@ def test_reference_inventory_pads_to_smooth_length():
 
 
 def test_hip_inventory_structure():
-    """Every piece is a 256-byte multiple; the dominant terms are 16 B/voxel of state,
-    32 B per padded spectrum bin and 4 B/voxel per map; linear drops one map; record
-    buffers scale with harmonics as 16 B + 4 B per recorded voxel."""
+    """Every piece is a 256-byte multiple; the dominant terms are 4 B/voxel of pressure,
+    56 B per padded spectrum bin (seven spectra) and 4 B/voxel per map; linear drops one
+    map; record buffers scale with harmonics as 16 B + 4 B per recorded voxel."""
     kw = dict(n_harmonics=1, rec_elems=256**3, n_src=5000, max_periods=64)
     lin = planner.hip_inventory((256, 256, 256), nonlinear=False, **kw)
     nl = planner.hip_inventory((256, 256, 256), nonlinear=True, **kw)
     assert all(v % 256 == 0 for v in lin.breakdown.values())
     P = 256**3
-    assert lin.breakdown["state (p + u)"] == 16 * P
-    assert lin.breakdown["half spectra (4, padded rows)"] == 32 * 256 * (256 * 144 + 16)
+    assert lin.breakdown["pressure (p)"] == 4 * P
+    assert lin.breakdown["half spectra (sp, sg0-2, velocity state su0-2)"] == 56 * 256 * (256 * 144 + 16)
     assert nl.total_bytes - lin.total_bytes == 4 * P
     h3 = planner.hip_inventory((256, 256, 256), nonlinear=False,
                                **{**kw, "n_harmonics": 3})
@@ -94,7 +94,7 @@ def test_mi355x_row_and_aliases():
     grid, med, src = tiny_setup()
     est = planner.estimate(grid, med, src, gpu="gfx950")
     assert est.gpu == "MI355X" and est.source == "db" and est.fits
-    assert "half spectra (4, padded rows)" in est.vram_breakdown
+    assert "half spectra (sp, sg0-2, velocity state su0-2)" in est.vram_breakdown
     assert est.vram_bytes_reference_model > 0
     other = planner.estimate(grid, med, src, gpu="A100")
     assert other.gpu == "A100-40GB" and "fft workspace" in other.vram_breakdown
@@ -106,9 +106,9 @@ def test_mi355x_row_and_aliases():
 
 def test_bandwidth_time_model():
     """db step time = step bytes / (datasheet bandwidth x bw_eff); 256^3 linear moves
-    224 H + 48 P bytes (2.97 GB) per step."""
+    248 H + 24 P bytes (2.50 GB) per step."""
     b = planner.step_bytes((256, 256, 256), False)
-    assert b == 224 * 256 * 256 * 129 + 48 * 256**3
+    assert b == 248 * 256 * 256 * 129 + 24 * 256**3
     grid = Grid((256, 256, 256), 2e-4, PMLSpec(thickness=16 * 2e-4))
     med = Medium.homogeneous(grid.shape, water())
     src = plane_cw_source(grid, f0=1e6, amplitude=1e5, axis=2)
