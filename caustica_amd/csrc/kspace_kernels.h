@@ -701,6 +701,45 @@ struct ZCount {
   static constexpr int SPT = (ZPlan<M, ZV>::LPB * (M + 1) + ZPlan<M, ZV>::THREADS - 1) / ZPlan<M, ZV>::THREADS;
 };
 
+// (x, y) of line line0 + l of a block. Lines of a block are consecutive, so when the
+// block holds at most ny lines they cross at most one x plane: one compare instead of a
+// per-thread integer division. x0, y0 (the block's first line) are block-uniform.
+template <int LPB>
+__device__ __forceinline__ void line_xy(int x0, int y0, int l, const Shape& sh, int& x, int& y) {
+  y = y0 + l;
+  x = x0;
+  if (LPB <= sh.ny) {
+    const bool w = y >= sh.ny;
+    y -= w ? sh.ny : 0;
+    x += w ? 1 : 0;
+  } else {
+    x += y / sh.ny;
+    y -= (y / sh.ny) * sh.ny;
+  }
+}
+
+// n / D for 0 <= n < NMAX as one full-rate 24-bit multiply and a shift (the compiler's
+// division by a constant uses a quarter-rate 32-bit high multiply). The multiplier is
+// checked exhaustively at compile time; when no exact one fits, plain division is used.
+template <int D, int NMAX>
+struct Div24 {
+  static constexpr unsigned S = 23;
+  static constexpr unsigned MG = (1u << S) / D + 1;
+  static constexpr bool exact() {
+    if (MG >= (1u << 24) || NMAX >= (1 << 24)) return false;
+    for (unsigned n = 0; n < (unsigned)NMAX; ++n) {
+      const unsigned long long p = (unsigned long long)n * MG;
+      if (p >= (1ull << 32) || (unsigned)(p >> S) != n / D) return false;
+    }
+    return true;
+  }
+  static constexpr bool OK = exact();
+  __device__ __forceinline__ static int div(int n) {
+    if constexpr (OK) return (int)(__umul24((unsigned)n, MG) >> S);
+    else return n / D;
+  }
+};
+
 __device__ __forceinline__ long row_of(long line, const Shape& s) {
   const long x = line / s.ny;
   return x * s.ps + (line - x * s.ny) * s.ms;
@@ -734,8 +773,9 @@ __device__ __forceinline__ void z_stage_load(const float2* twg, const float2* rt
     st.sz[q] = szg[im];
     st.rt[q] = rtg[i <= M ? i : M];
   }
-  const int line = (int)line0 + ((int)threadIdx.x < nl ? (int)threadIdx.x : nl - 1);
-  const int x = line / sh.ny, y = line - x * sh.ny;
+  const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;  // block-uniform
+  int x, y;
+  line_xy<ZPlan<M, ZV>::LPB>(x0, y0, (int)threadIdx.x < nl ? (int)threadIdx.x : nl - 1, sh, x, y);
   const gptr<const float> sxg = G(m.sponge) + 2 * M;
   st.sx = sxg[x];
   st.sy = sxg[sh.nx + y];
@@ -755,8 +795,9 @@ __device__ __forceinline__ void z_stage_put(const ZStage<M, ZV>& st, const Shape
     if (i <= M) s.rt[i] = st.rt[q];
   }
   if (threadIdx.x < nl) {
-    const int line = (int)line0 + threadIdx.x;
-    const int x = line / sh.ny, y = line - x * sh.ny;
+    const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;
+    int x, y;
+    line_xy<ZPlan<M, ZV>::LPB>(x0, y0, threadIdx.x, sh, x, y);
     s.sxy[threadIdx.x] = st.sx * st.sy;
     s.lx[threadIdx.x] = x;
     s.ly[threadIdx.x] = y;
@@ -769,24 +810,20 @@ template <int M, int ZV>
 __device__ __forceinline__ void z_spec_load(gptr<const float2> spec, long line0, int nl,
                                             const Shape& sh, float2 (&r)[ZCount<M, ZV>::SPT]) {
   constexpr int NB = M + 1;
-  const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;
-  // lines of a block are consecutive; with LPB <= ny they cross at most one x plane
-  const bool one_wrap = ZPlan<M, ZV>::LPB <= sh.ny;
+  const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;  // block-uniform
+  // row of line (x, y) = x ps + y ms = line ms + 16 x (ps = ny ms + 16): offsets from the
+  // block's first row fit 32 bits and need no per-element 64-bit multiply
+  const gptr<const float2> base = spec + ((long)line0 * sh.ms + 16L * x0);
 #pragma unroll
   for (int j = 0; j < ZCount<M, ZV>::SPT; ++j) {
     const int i0 = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
     const int i = i0 < nl * NB ? i0 : nl * NB - 1;  // unconditional load (see z_stage_load)
-    const int l = i / NB, k = i - l * NB;
-    int y = y0 + l, x = x0;
-    if (one_wrap) {
-      const bool w = y >= sh.ny;
-      y -= w ? sh.ny : 0;
-      x += w ? 1 : 0;
-    } else {
-      x += y / sh.ny;
-      y -= (y / sh.ny) * sh.ny;
-    }
-    r[j] = spec[(long)x * sh.ps + (long)y * sh.ms + k];
+    const int l = Div24<NB, ZPlan<M, ZV>::LPB * NB>::div(i), k = i - l * NB;
+    int x, y;
+    line_xy<ZPlan<M, ZV>::LPB>(x0, y0, l, sh, x, y);
+    // 24-bit multiply and unsigned sums: the offset stays a full-rate 32-bit computation
+    const unsigned off = __umul24((unsigned)l, (unsigned)sh.ms) + 16u * (unsigned)(x - x0) + (unsigned)k;
+    r[j] = base[off];
   }
 }
 
@@ -797,7 +834,7 @@ __device__ __forceinline__ void z_spec_put(int nl, const float2 (&r)[ZCount<M, Z
   for (int j = 0; j < ZCount<M, ZV>::SPT; ++j) {
     const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
     if (i < nl * NB) {
-      const int l = i / NB, k = i - l * NB;
+      const int l = Div24<NB, ZPlan<M, ZV>::LPB * NB>::div(i), k = i - l * NB;
       s.a[l * ZPlan<M, ZV>::LSA + k] = r[j];
     }
   }
