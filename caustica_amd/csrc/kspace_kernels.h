@@ -221,6 +221,62 @@ __device__ __forceinline__ float sinc_q(float q) {
 
 // ================================================================ y pass
 // In-place C2C along y of one or three half spectra. Block = (x, column tile of TC bins).
+// Column geometry of a strided-pass block (the spectrum layout is described at Shape).
+// Main blocks take TC consecutive bins k < nz/2 of one plane (y pass: an x plane,
+// stride ms along y; x pass: a y row, stride ps along x); the last
+// ceil(planes / TC) blocks take the Nyquist bin k = nz/2 of TC consecutive planes from
+// the side plane. Invalid columns address a valid element and are never stored.
+struct Col {
+  long base, stride;
+  int plane, zb;
+  bool valid;
+};
+__host__ __device__ constexpr int strided_blocks(int planes, int nzh, int tc) {
+  return planes * ((nzh - 1 + tc - 1) / tc) + (planes + tc - 1) / tc;
+}
+template <int TC>
+__device__ __forceinline__ Col y_column(const Shape& s, int col) {
+  const int M = s.nzh - 1, nt = (M + TC - 1) / TC, b = blockIdx.x;
+  Col c;
+  if (b < s.nx * nt) {
+    c.plane = b / nt;
+    const int zb0 = (b - c.plane * nt) * TC + col;
+    c.valid = zb0 < M;
+    c.zb = c.valid ? zb0 : 0;
+    c.base = (long)c.plane * s.ps + c.zb;
+    c.stride = s.ms;
+  } else {
+    const int x0 = (b - s.nx * nt) * TC + col;
+    c.valid = x0 < s.nx;
+    c.plane = c.valid ? x0 : 0;
+    c.zb = M;
+    c.base = s.so + (long)c.plane * s.ny;
+    c.stride = 1;
+  }
+  return c;
+}
+template <int TC>
+__device__ __forceinline__ Col x_column(const Shape& s, int col) {
+  const int M = s.nzh - 1, nt = (M + TC - 1) / TC, b = blockIdx.x;
+  Col c;
+  if (b < s.ny * nt) {
+    c.plane = b / nt;
+    const int zb0 = (b - c.plane * nt) * TC + col;
+    c.valid = zb0 < M;
+    c.zb = c.valid ? zb0 : 0;
+    c.base = (long)c.plane * s.ms + c.zb;
+    c.stride = s.ps;
+  } else {
+    const int y0 = (b - s.ny * nt) * TC + col;
+    c.valid = y0 < s.ny;
+    c.plane = c.valid ? y0 : 0;
+    c.zb = M;
+    c.base = s.so + c.plane;
+    c.stride = s.ny;
+  }
+  return c;
+}
+
 template <int L, int DIR>
 __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
     k_ypass(StepArgs a, const float2* __restrict__ twg, int which) {
@@ -235,12 +291,10 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
   const gptr<float2> data = G(which == 0 ? m.sp : m.sg[blockIdx.y]);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
-  const int nt = (a.s.nzh + TC - 1) / TC;
-  const int x = blockIdx.x / nt;
-  const int zb = (blockIdx.x - x * nt) * TC + col;
-  const bool valid = zb < a.s.nzh;
-  const long S_ = a.s.ms;
-  const long base = (long)x * a.s.ps + (valid ? zb : 0);
+  const Col cg = y_column<TC>(a.s, col);
+  const bool valid = cg.valid;
+  const long S_ = cg.stride;
+  const long base = cg.base;
   float2 V[S::R1][S::B];
   // invalid columns load bin 0 of their row (a valid address) and are never stored:
   // unconditional loads let the wait-count pass wait for exactly what is consumed
@@ -326,13 +380,11 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   KyStage<L, TC * T> kys;
   kys.load(m, a.s);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
-  const int nt = (a.s.nzh + TC - 1) / TC;
-  const int x = blockIdx.x / nt;
-  const int zb0 = (blockIdx.x - x * nt) * TC + col;
-  const bool valid = zb0 < a.s.nzh;
-  const int zb = valid ? zb0 : 0;
-  const long S_ = a.s.ms;
-  const long base = (long)x * a.s.ps + zb;
+  const Col cg = y_column<TC>(a.s, col);
+  const bool valid = cg.valid;
+  const int zb = cg.zb;
+  const long S_ = cg.stride;
+  const long base = cg.base;
   const gptr<float2> g0 = G(m.sg[0]), g1 = G(m.sg[1]), g2 = G(m.sg[2]);
   float2 VA[S::R1][S::B], VQ[S::R1][S::B];
   in_load<L, T>(t, [&](int n) { return ld(g0, base + n * S_); }, VA);  // see k_ypass
@@ -387,13 +439,11 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   KyStage<L, TC * T> kys;
   kys.load(m, a.s);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
-  const int nt = (a.s.nzh + TC - 1) / TC;
-  const int x = blockIdx.x / nt;
-  const int zb0 = (blockIdx.x - x * nt) * TC + col;
-  const bool valid = zb0 < a.s.nzh;
-  const int zb = valid ? zb0 : 0;
-  const long S_ = a.s.ms;
-  const long base = (long)x * a.s.ps + zb;
+  const Col cg = y_column<TC>(a.s, col);
+  const bool valid = cg.valid;
+  const int zb = cg.zb;
+  const long S_ = cg.stride;
+  const long base = cg.base;
   const gptr<float2> u0 = G(m.su[0]), u1 = G(m.su[1]), u2 = G(m.su[2]);
   const gptr<float2> g0 = G(m.sg[0]), g1 = G(m.sg[1]);
   float2 VX[S::R1][S::B], VY[S::R1][S::B], VZ[S::R1][S::B];
@@ -506,13 +556,11 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   KxStage<L, TC * T> kxs;
   kxs.load(m);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
-  const int nt = (a.s.nzh + TC - 1) / TC;
-  const int y = blockIdx.x / nt;
-  const int zb0 = (blockIdx.x - y * nt) * TC + col;
-  const bool valid = zb0 < a.s.nzh;
-  const int zb = valid ? zb0 : 0;
-  const long C = a.s.ps;
-  const long base = (long)y * a.s.ms + zb;
+  const Col cg = x_column<TC>(a.s, col);
+  const bool valid = cg.valid;
+  const int y = cg.plane, zb = cg.zb;
+  const long C = cg.stride;
+  const long base = cg.base;
   float2 V[S::R1][S::B];
   const gptr<const float2> src = G(m.sp);
   in_load<L, T>(t, [&](int n) { return ld(src, base + n * C); }, V);  // see k_ypass
@@ -588,13 +636,11 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   KxStage<L, TC * T> kxs;
   kxs.load(m);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
-  const int nt = (a.s.nzh + TC - 1) / TC;
-  const int y = blockIdx.x / nt;
-  const int zb0 = (blockIdx.x - y * nt) * TC + col;
-  const bool valid = zb0 < a.s.nzh;
-  const int zb = valid ? zb0 : 0;
-  const long C = a.s.ps;
-  const long base = (long)y * a.s.ms + zb;
+  const Col cg = x_column<TC>(a.s, col);
+  const bool valid = cg.valid;
+  const int y = cg.plane, zb = cg.zb;
+  const long C = cg.stride;
+  const long base = cg.base;
   float2 V[S::R1][S::B], W[S::R1][S::B];
   const gptr<const float2> ux = G(m.sg[0]), uw = G(m.sg[1]);
   in_load<L, T>(t, [&](int n) { return ld(ux, base + n * C); }, V);  // see k_ypass
@@ -694,11 +740,10 @@ struct ZLds {
   int sptr[ZPlan<M, ZV>::LPB + 1];  // source CSR bounds of the block's lines
 };
 
-// per-thread counts of the block's contiguous real chunk (float2) and spectrum bins
+// per-thread count of the block's contiguous real chunk (float2) and of its row bins
 template <int M, int ZV>
 struct ZCount {
   static constexpr int EPT = (ZPlan<M, ZV>::LPB * M + ZPlan<M, ZV>::THREADS - 1) / ZPlan<M, ZV>::THREADS;
-  static constexpr int SPT = (ZPlan<M, ZV>::LPB * (M + 1) + ZPlan<M, ZV>::THREADS - 1) / ZPlan<M, ZV>::THREADS;
 };
 
 // (x, y) of line line0 + l of a block. Lines of a block are consecutive, so when the
@@ -739,11 +784,6 @@ struct Div24 {
     else return n / D;
   }
 };
-
-__device__ __forceinline__ long row_of(long line, const Shape& s) {
-  const long x = line / s.ny;
-  return x * s.ps + (line - x * s.ny) * s.ms;
-}
 
 // Tables a block stages into LDS (twiddles, real-split factors, z sponge, per-line
 // sponge product and spectrum-row offsets). They are loaded into registers first and
@@ -805,39 +845,43 @@ __device__ __forceinline__ void z_stage_put(const ZStage<M, ZV>& st, const Shape
   }
 }
 
-// issue the loads of the block's spectrum rows (M + 1 bins per line) into registers
+// issue the loads of the block's spectra into registers: bins k < M of each line from its
+// row (thread-to-element mapping of the real volumes), and the Nyquist bin k = M of line
+// line0 + threadIdx.x from the side plane
 template <int M, int ZV>
 __device__ __forceinline__ void z_spec_load(gptr<const float2> spec, long line0, int nl,
-                                            const Shape& sh, float2 (&r)[ZCount<M, ZV>::SPT]) {
-  constexpr int NB = M + 1;
+                                            const Shape& sh, float2 (&r)[ZCount<M, ZV>::EPT],
+                                            float2& rn) {
   const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;  // block-uniform
   // row of line (x, y) = x ps + y ms = line ms + 16 x (ps = ny ms + 16): offsets from the
   // block's first row fit 32 bits and need no per-element 64-bit multiply
   const gptr<const float2> base = spec + ((long)line0 * sh.ms + 16L * x0);
 #pragma unroll
-  for (int j = 0; j < ZCount<M, ZV>::SPT; ++j) {
+  for (int j = 0; j < ZCount<M, ZV>::EPT; ++j) {
     const int i0 = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
-    const int i = i0 < nl * NB ? i0 : nl * NB - 1;  // unconditional load (see z_stage_load)
-    const int l = Div24<NB, ZPlan<M, ZV>::LPB * NB>::div(i), k = i - l * NB;
+    const int i = i0 < nl * M ? i0 : nl * M - 1;  // unconditional load (see z_stage_load)
+    const int l = Div24<M, ZPlan<M, ZV>::LPB * M>::div(i), k = i - l * M;
     int x, y;
     line_xy<ZPlan<M, ZV>::LPB>(x0, y0, l, sh, x, y);
     // 24-bit multiply and unsigned sums: the offset stays a full-rate 32-bit computation
     const unsigned off = __umul24((unsigned)l, (unsigned)sh.ms) + 16u * (unsigned)(x - x0) + (unsigned)k;
     r[j] = base[off];
   }
+  rn = spec[sh.so + line0 + ((int)threadIdx.x < nl ? (int)threadIdx.x : nl - 1)];
 }
 
 template <int M, int ZV>
-__device__ __forceinline__ void z_spec_put(int nl, const float2 (&r)[ZCount<M, ZV>::SPT], ZLds<M, ZV>& s) {
-  constexpr int NB = M + 1;
+__device__ __forceinline__ void z_spec_put(int nl, const float2 (&r)[ZCount<M, ZV>::EPT], float2 rn,
+                                           ZLds<M, ZV>& s) {
 #pragma unroll
-  for (int j = 0; j < ZCount<M, ZV>::SPT; ++j) {
+  for (int j = 0; j < ZCount<M, ZV>::EPT; ++j) {
     const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
-    if (i < nl * NB) {
-      const int l = Div24<NB, ZPlan<M, ZV>::LPB * NB>::div(i), k = i - l * NB;
+    if (i < nl * M) {
+      const int l = Div24<M, ZPlan<M, ZV>::LPB * M>::div(i), k = i - l * M;
       s.a[l * ZPlan<M, ZV>::LSA + k] = r[j];
     }
   }
+  if ((int)threadIdx.x < nl) s.a[threadIdx.x * ZPlan<M, ZV>::LSA + M] = rn;
 }
 
 // issue the loads of a real volume's chunk (float2 stream) into registers
@@ -878,10 +922,11 @@ __device__ __forceinline__ void z_c2r(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   __syncthreads();
 }
 
-// R2C of the real line in a (M float2) -> spectrum row in HBM.
+// R2C of the real line in a (M float2) -> spectrum row in HBM (bins k < M) and the
+// line's Nyquist entry of the side plane (dst_nyq).
 template <int M, int ZV>
 __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s,
-                                      gptr<float2> dst_row) {
+                                      gptr<float2> dst_row, gptr<float2> dst_nyq) {
   using Z = ZPlan<M, ZV>;
   using S = Split<M, Z::ZT>;
   float2* la = s.a + ll * Z::LSA;
@@ -904,7 +949,8 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   }
   __syncthreads();
   if (lvalid) {
-    for (int k = t; k <= M; k += Z::ZT) dst_row[k] = r2c_post<M>(la, k, s.rt);
+    for (int k = t; k < M; k += Z::ZT) dst_row[k] = r2c_post<M>(la, k, s.rt);
+    if (t == 0) *dst_nyq = r2c_post<M>(la, M, s.rt);
   }
 }
 
@@ -943,13 +989,13 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   // spectrum, then the maps
   ZStage<M, ZV> stg;
   z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
-  float2 sr[CN::SPT], ur[CN::SPT], dto[CN::EPT], ab[CN::EPT];
-  z_spec_load<M, ZV>(gspec, line0, nl, a.s, sr);
-  z_spec_load<M, ZV>(uspec, line0, nl, a.s, ur);
+  float2 sr[CN::EPT], ur[CN::EPT], dto[CN::EPT], ab[CN::EPT], srn, urn;
+  z_spec_load<M, ZV>(gspec, line0, nl, a.s, sr, srn);
+  z_spec_load<M, ZV>(uspec, line0, nl, a.s, ur, urn);
   z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.dto_rho)) + e0, nl, dto);
   z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.absorb)) + e0, nl, ab);
   z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
-  z_spec_put<M, ZV>(nl, sr, s);
+  z_spec_put<M, ZV>(nl, sr, srn, s);
   __syncthreads();
   z_c2r<M, ZV>(ll, t, lvalid, s);
   float2 gg[CN::EPT];
@@ -957,18 +1003,18 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   for (int j = 0; j < CN::EPT; ++j) {
     const int i = threadIdx.x + j * Z::THREADS;
     const int ic = i < nl * M ? i : nl * M - 1;
-    const int l = ic / M, mm = ic - l * M;
+    const int l = Div24<M, Z::LPB * M>::div(ic), mm = ic - l * M;
     gg[j] = s.a[l * Z::LSA + mm];
   }
   __syncthreads();
-  z_spec_put<M, ZV>(nl, ur, s);
+  z_spec_put<M, ZV>(nl, ur, urn, s);
   __syncthreads();
   z_c2r<M, ZV>(ll, t, lvalid, s);
 #pragma unroll
   for (int j = 0; j < CN::EPT; ++j) {
     const int i = threadIdx.x + j * Z::THREADS;
     if (i < nl * M) {
-      const int l = i / M, mm = i - l * M;
+      const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
       const float sxy = s.sxy[l];
       const float2 s2 = s.sz[mm];
       float2 uv = s.a[l * Z::LSA + mm];
@@ -984,7 +1030,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     }
   }
   __syncthreads();
-  z_r2c<M, ZV>(ll, t, lvalid, s, uspec + s.row[lvalid ? ll : 0]);
+  z_r2c<M, ZV>(ll, t, lvalid, s, uspec + s.row[lvalid ? ll : 0], uspec + a.s.so + line0 + (lvalid ? ll : 0));
 }
 
 // Conversion of one velocity component between its state form and a real volume, for
@@ -1015,25 +1061,25 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     for (int j = 0; j < CN::EPT; ++j) {
       const int i = threadIdx.x + j * Z::THREADS;
       if (i < nl * M) {
-        const int l = i / M, mm = i - l * M;
+        const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
         s.a[l * Z::LSA + mm] = vv[j];
       }
     }
     __syncthreads();
-    z_r2c<M, ZV>(ll, t, lvalid, s, uspec + s.row[lvalid ? ll : 0]);
+    z_r2c<M, ZV>(ll, t, lvalid, s, uspec + s.row[lvalid ? ll : 0], uspec + a.s.so + line0 + (lvalid ? ll : 0));
   } else {
     const float inz = m.inv_nz;
-    float2 sr[CN::SPT];
-    z_spec_load<M, ZV>(uspec, line0, nl, a.s, sr);
+    float2 sr[CN::EPT], srn;
+    z_spec_load<M, ZV>(uspec, line0, nl, a.s, sr, srn);
     z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
-    z_spec_put<M, ZV>(nl, sr, s);
+    z_spec_put<M, ZV>(nl, sr, srn, s);
     __syncthreads();
     z_c2r<M, ZV>(ll, t, lvalid, s);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
       const int i = threadIdx.x + j * Z::THREADS;
       if (i < nl * M) {
-        const int l = i / M, mm = i - l * M;
+        const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
         const float2 v = s.a[l * Z::LSA + mm];
         real[i] = make_float2(v.x * inz, v.y * inz);
       }
@@ -1072,8 +1118,8 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     // before the C2R), then the volumes (after it: their latency overlaps the transform)
     ZStage<M, ZV> stg;
     z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
-    float2 sr[CN::SPT], rc[CN::EPT], ab[CN::EPT], bt[NL ? CN::EPT : 1];
-    z_spec_load<M, ZV>(G(m.sp), line0, nl, a.s, sr);
+    float2 sr[CN::EPT], rc[CN::EPT], ab[CN::EPT], bt[NL ? CN::EPT : 1], srn;
+    z_spec_load<M, ZV>(G(m.sp), line0, nl, a.s, sr, srn);
     const gptr<const int> sptr = G(m.src_ptr);
     const int sbeg = sptr[line0], send = sptr[line0 + nl];
     const bool src_fit = send - sbeg <= kSrcCap;  // block-uniform
@@ -1105,7 +1151,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
         src_val[j - sbeg] = sm.amp * (sw_pf[q].x * sm.sin_t - sw_pf[q].y * sm.cos_t);
       }
     }
-    z_spec_put<M, ZV>(nl, sr, s);
+    z_spec_put<M, ZV>(nl, sr, srn, s);
     __syncthreads();
     z_c2r<M, ZV>(ll, t, lvalid, s);
     // descriptor fields used after the first store, read once here: a global store may
@@ -1169,7 +1215,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     for (int j = 0; j < CN::EPT; ++j) {
       const int i = threadIdx.x + j * Z::THREADS;
       if (i < nl * M) {
-        const int l = i / M, mm = i - l * M;
+        const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
         const float sxy = s.sxy[l];
         const float2 dv = s.a[l * Z::LSA + mm];
         float2 pv = pp[j];
@@ -1215,7 +1261,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
       __syncthreads();
 #pragma unroll 1
       for (int i = threadIdx.x; i < nl * M; i += Z::THREADS) {
-        const int l = i / M, mm = i - l * M;
+        const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
         finish(i, l, mm, s.a[l * Z::LSA + mm]);
       }
     }
@@ -1236,13 +1282,13 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     for (int j = 0; j < CN::EPT; ++j) {
       const int i = threadIdx.x + j * Z::THREADS;
       if (i < nl * M) {
-        const int l = i / M, mm = i - l * M;
+        const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
         s.a[l * Z::LSA + mm] = pp[j];
       }
     }
   }
   __syncthreads();
-  z_r2c<M, ZV>(ll, t, lvalid, s, G(m.sp) + s.row[lvalid ? ll : 0]);
+  z_r2c<M, ZV>(ll, t, lvalid, s, G(m.sp) + s.row[lvalid ? ll : 0], G(m.sp) + a.s.so + line0 + (lvalid ? ll : 0));
 }
 
 }  // namespace cst

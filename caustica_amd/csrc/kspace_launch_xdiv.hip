@@ -9,7 +9,7 @@ bool launch_xdiv(int L, const StepArgs& a, const float2* tw, hipStream_t st) {
 #define CST_XD(N)                                                                \
   case N: {                                                                      \
     using P = Plan<N>;                                                           \
-    dim3 g((unsigned)(a.s.ny * ((a.s.nzh + P::TC - 1) / P::TC)), 1, a.nmem);     \
+    dim3 g((unsigned)strided_blocks(a.s.ny, a.s.nzh, P::TC), 1, a.nmem);         \
     hipLaunchKernelGGL((k_xdiv<N>), g, dim3(P::TC * P::T), 0, st, a, tw);        \
     return true;                                                                 \
   }

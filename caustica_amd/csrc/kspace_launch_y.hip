@@ -10,7 +10,7 @@ bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, hi
 #define CST_Y(N)                                                                            \
   case N: {                                                                                 \
     using P = Plan<N>;                                                                      \
-    dim3 g(a.s.nx * ((a.s.nzh + P::TC - 1) / P::TC), ncomp, a.nmem);                        \
+    dim3 g(strided_blocks(a.s.nx, a.s.nzh, P::TC), ncomp, a.nmem);                           \
     if (dir < 0)                                                                            \
       hipLaunchKernelGGL((k_ypass<N, -1>), g, dim3(P::TC * P::T), 0, st, a, tw, which);     \
     else                                                                                    \
@@ -36,7 +36,7 @@ bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, hi
 #define CST_Y3_CASE(N)                                                              \
   case N: {                                                                         \
     using P = Plan<N>;                                                              \
-    dim3 g(a.s.nx * ((a.s.nzh + P::TC - 1) / P::TC), 1, a.nmem);                     \
+    dim3 g(strided_blocks(a.s.nx, a.s.nzh, P::TC), 1, a.nmem);                        \
     hipLaunchKernelGGL((CST_Y3_KERNEL<N>), g, dim3(P::TC * P::T), 0, st, a, tw);     \
     return true;                                                                    \
   }

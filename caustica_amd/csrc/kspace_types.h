@@ -84,10 +84,17 @@ struct StepMember {
   int pad_;
 };
 
+// Half-spectrum layout (nzh = nz/2 + 1 bins per (x, y)): bins k < nz/2 of (x, y) sit in
+// rows at x ps + y ms + k; the Nyquist bin k = nz/2 of every (x, y) sits in a side plane
+// at so + x ny + y. Rows thus hold a multiple of 16 bins (128 B) and every column tile
+// of the strided passes is a full, aligned cache line; the side plane is one more small
+// column for them.
 struct Shape {
   int nx, ny, nz, nzh;
-  int ms;   // spectrum row stride in bins: nzh rounded up to a multiple of 16
-  long ps;  // spectrum x-plane stride in bins: ny * ms + 16 (breaks power-of-two strides)
+  int ms;   // row stride in bins: nz/2 rounded up to a multiple of 16
+  long ps;  // x-plane stride in bins: ny * ms + 16 (breaks power-of-two strides)
+  long so;  // offset of the Nyquist side plane: nx * ps
+  long hs;  // bins per spectrum array: so + nx * ny rounded up to a multiple of 16
 };
 
 struct StepArgs {

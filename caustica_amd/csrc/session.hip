@@ -215,8 +215,9 @@ int cst_session_create(int nx, int ny, int nz, int max_members, int device, cst_
   auto* s = new cst_session();
   s->device = device;
   {
-    const int ms = ((nz / 2 + 1 + 15) / 16) * 16;
-    s->s = Shape{nx, ny, nz, nz / 2 + 1, ms, (long)ny * ms + 16};
+    const int ms = ((nz / 2 + 15) / 16) * 16;
+    const long ps = (long)ny * ms + 16, so = (long)nx * ps;
+    s->s = Shape{nx, ny, nz, nz / 2 + 1, ms, ps, so, so + (((long)nx * ny + 15) / 16) * 16};
   }
   s->max_members = max_members;
   s->members.resize(max_members);
@@ -327,7 +328,7 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
 
   // one device allocation, carved in 256-byte aligned pieces
   const int64_t P = (int64_t)sh.nx * sh.ny * sh.nz;
-  const int64_t H = (int64_t)sh.nx * sh.ps;
+  const int64_t H = sh.hs;  // bins per spectrum array (rows + Nyquist side plane)
   const int64_t R = (int64_t)(d->rec[1] - d->rec[0]) * (d->rec[3] - d->rec[2]) * (d->rec[5] - d->rec[4]);
   const int nmaps = d->beta2_dt ? 4 : 3;
   const int64_t nk = 2LL * (sh.nx + sh.ny + sh.nzh);
@@ -497,7 +498,7 @@ int cst_member_reset(cst_session* s, int slot) {
   Member& m = s->members[slot];
   const Shape& sh = s->s;
   const int64_t P = (int64_t)sh.nx * sh.ny * sh.nz;
-  const int64_t H = (int64_t)sh.nx * sh.ps;
+  const int64_t H = sh.hs;
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipMemsetAsync(m.dev.p, 0, P * 4, s->stream));
   HIPCHK(hipMemsetAsync(m.dev.sp, 0, 7 * H * 8, s->stream));
@@ -668,7 +669,7 @@ int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, co
 
 static size_t raw_state_bytes(const cst_session* s) {
   const Shape& sh = s->s;
-  return (size_t)sh.nx * sh.ny * sh.nz * 4 + 3 * (size_t)sh.nx * sh.ps * 8;
+  return (size_t)sh.nx * sh.ny * sh.nz * 4 + 3 * (size_t)sh.hs * 8;
 }
 
 int cst_member_state_size(cst_session* s, size_t* bytes) {
@@ -681,7 +682,7 @@ int cst_member_get_raw(cst_session* s, int slot, void* buf, size_t bytes, int64_
   if (int r = check_slot(s, slot)) return r;
   if (!buf || bytes != raw_state_bytes(s)) return fail(CST_ERR_ARG, "buffer size differs from cst_member_state_size");
   Member& m = s->members[slot];
-  const size_t pb = (size_t)s->s.nx * s->s.ny * s->s.nz * 4, hb = (size_t)s->s.nx * s->s.ps * 8;
+  const size_t pb = (size_t)s->s.nx * s->s.ny * s->s.nz * 4, hb = (size_t)s->s.hs * 8;
   HIPCHK(hipSetDevice(s->device));
   char* dst = static_cast<char*>(buf);
   HIPCHK(hipMemcpyAsync(dst, m.dev.p, pb, hipMemcpyDeviceToHost, s->stream));
@@ -697,7 +698,7 @@ int cst_member_set_raw(cst_session* s, int slot, const void* buf, size_t bytes, 
   if (!buf || bytes != raw_state_bytes(s)) return fail(CST_ERR_ARG, "buffer size differs from cst_member_state_size");
   if (n < 0) return fail(CST_ERR_ARG, "negative step");
   Member& m = s->members[slot];
-  const size_t pb = (size_t)s->s.nx * s->s.ny * s->s.nz * 4, hb = (size_t)s->s.nx * s->s.ps * 8;
+  const size_t pb = (size_t)s->s.nx * s->s.ny * s->s.nz * 4, hb = (size_t)s->s.hs * 8;
   HIPCHK(hipSetDevice(s->device));
   const char* src = static_cast<const char*>(buf);
   HIPCHK(hipMemcpyAsync(m.dev.p, src, pb, hipMemcpyHostToDevice, s->stream));

@@ -137,9 +137,9 @@ def hip_inventory(padded3, *, nonlinear: bool, n_harmonics: int, rec_elems: int,
     """Bytes of one member on the hip engine, piece by piece as the library carves them."""
     nx, ny, nz = (int(v) for v in padded3)
     nzh = nz // 2 + 1
-    ms = (nzh + 15) // 16 * 16
+    ms = (nz // 2 + 15) // 16 * 16
     P = nx * ny * nz
-    H = nx * (ny * ms + 16)
+    H = nx * (ny * ms + 16) + (nx * ny + 15) // 16 * 16  # rows + Nyquist side plane
     ns = max(int(n_src), 1)
     b = {
         "pressure (p)": _al(P * 4),

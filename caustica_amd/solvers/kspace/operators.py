@@ -175,7 +175,8 @@ class MemberPlan:
         """HBM held by the member on the device (admission footprint)."""
         nx, ny, nz = self.padded3
         P = nx * ny * nz
-        H = nx * (ny * ((nz // 2 + 1 + 15) // 16 * 16) + 16)  # padded spectrum rows/planes
+        # one spectrum: rows of nz/2 bins padded to 16, planes + 16, Nyquist side plane
+        H = nx * (ny * ((nz // 2 + 15) // 16 * 16) + 16) + (nx * ny + 15) // 16 * 16
         R = int(np.prod(self.rec_shape))
         nmaps = 3 + (self.beta2_dt is not None)
         return int(4 * P + 56 * H + 4 * nmaps * P + R * (16 * len(self.harmonics) + 4) + (1 << 20))

@@ -55,7 +55,7 @@ def test_hip_inventory_structure():
     assert all(v % 256 == 0 for v in lin.breakdown.values())
     P = 256**3
     assert lin.breakdown["pressure (p)"] == 4 * P
-    assert lin.breakdown["half spectra (sp, sg0-2, velocity state su0-2)"] == 56 * 256 * (256 * 144 + 16)
+    assert lin.breakdown["half spectra (sp, sg0-2, velocity state su0-2)"] == 56 * (256 * (256 * 128 + 16) + 256 * 256)
     assert nl.total_bytes - lin.total_bytes == 4 * P
     h3 = planner.hip_inventory((256, 256, 256), nonlinear=False,
                                **{**kw, "n_harmonics": 3})
