@@ -960,7 +960,7 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
 // 24 H bytes instead of 16 H + 8 P. The block C2Rs the gradient spectrum, keeps the
 // real gradient in registers, C2Rs the velocity spectrum, updates
 // u = ((u - dt/rho g) absorb) sponge in LDS and R2Cs it back into su.
-// Block b covers line group g and axis ax with b = 24 * (g / 8) + 8 * ax + (g % 8):
+// Block b covers line group ngroups - 1 - g and axis ax with b = 24 * (g / 8) + 8 * ax + (g % 8):
 // blocks are dealt to the 8 XCDs round-robin, so the three axis blocks of a line group
 // run back to back on one XCD and its L2 serves the second and third reads of dt/rho
 // and absorb.
@@ -974,9 +974,12 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   const int nlines = a.s.nx * a.s.ny;
   const int ngroups = (nlines + Z::LPB - 1) / Z::LPB;
   const int b = blockIdx.x;
-  const int g = (b / 24) * 8 + (b % 8);
+  const int g0 = (b / 24) * 8 + (b % 8);
   const int ax = (b / 8) % 3;
-  if (g >= ngroups) return;
+  if (g0 >= ngroups) return;
+  // line groups in reverse: the y pass before this one wrote the spectra in increasing x,
+  // so the lines it wrote last (still in the last-level cache) are read first
+  const int g = ngroups - 1 - g0;
   const long line0 = (long)g * Z::LPB;
   const int nl = cmin(Z::LPB, nlines - (int)line0);
   const int ll = threadIdx.x / Z::ZT, t = threadIdx.x % Z::ZT;
@@ -1106,7 +1109,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
   const StepMember& sm = a.sm[blockIdx.z];
   const int nlines = a.s.nx * a.s.ny;
-  const long line0 = (long)blockIdx.x * Z::LPB;
+  const long line0 = (long)(gridDim.x - 1 - blockIdx.x) * Z::LPB;  // reverse order, as in k_zu
   const int nl = cmin(Z::LPB, nlines - (int)line0);
   const int ll = threadIdx.x / Z::ZT, t = threadIdx.x % Z::ZT;
   const bool lvalid = ll < nl;
