@@ -954,6 +954,13 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   }
 }
 
+// Waves per SIMD requested for k_zu: four (128 VGPRs) for the layout-0 kernel (the LDS
+// of its 4-wave block allows four per SIMD), except at the lengths where the compiler
+// then spills to scratch; else no constraint.
+constexpr int zu_waves(int M, int ZV) {
+  return (ZV == 0 && M != 72 && M != 80 && M != 96 && M != 100 && M != 144 && M != 200) ? 4 : 1;
+}
+
 // velocity update, one axis per block. The velocity state is kept as its z-spectrum
 // (su), the form the divergence needs next, so a component is read and written once
 // per step as a spectrum instead of as a real volume plus its spectrum: per axis
@@ -966,6 +973,7 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
 // and absorb.
 template <int M, int ZV>
 __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
+    __attribute__((amdgpu_waves_per_eu(zu_waves(M, ZV))))
     k_zu(StepArgs a, const float2* __restrict__ twg, const float2* __restrict__ rtg) {
   using Z = ZPlan<M, ZV>;
   using CN = ZCount<M, ZV>;
@@ -989,18 +997,19 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   const gptr<float2> uspec = G(m.su[ax]);
   const float inz = m.inv_nz;
   // loads in the order they are consumed: staged tables, gradient spectrum, velocity
-  // spectrum, then the maps
+  // spectrum; the maps after the first transform (their registers are not held through
+  // it, which keeps the kernel at four waves per SIMD)
   ZStage<M, ZV> stg;
   z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
   float2 sr[CN::EPT], ur[CN::EPT], dto[CN::EPT], ab[CN::EPT], srn, urn;
   z_spec_load<M, ZV>(gspec, line0, nl, a.s, sr, srn);
   z_spec_load<M, ZV>(uspec, line0, nl, a.s, ur, urn);
-  z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.dto_rho)) + e0, nl, dto);
-  z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.absorb)) + e0, nl, ab);
   z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
   z_spec_put<M, ZV>(nl, sr, srn, s);
   __syncthreads();
   z_c2r<M, ZV>(ll, t, lvalid, s);
+  z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.dto_rho)) + e0, nl, dto);
+  z_vol_load<M, ZV>(G(reinterpret_cast<const float2*>(m.absorb)) + e0, nl, ab);
   float2 gg[CN::EPT];
 #pragma unroll
   for (int j = 0; j < CN::EPT; ++j) {
