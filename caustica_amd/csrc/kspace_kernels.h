@@ -59,15 +59,18 @@ struct Split {
 
 // z passes: complex length M = nz / 2; ZT threads per line, LPB lines per block.
 // Variant V = 0: ZT = max(A, B) (every lane busy in the larger stage); V = 1: twice the
-// threads per line (half the registers per thread). The session picks one per shape.
+// threads per line (half the registers per thread); V = 2 (velocity pass only): the
+// layout of V = 0 with half the lines per block. The session picks one per shape.
 template <int M, int V>
 struct ZPlan {
   static constexpr int A = Plan<M>::A;
   static constexpr int B = Plan<M>::B;
-  static constexpr int ZT = cmin(64, (V + 1) * cmax(A, B));
+  static constexpr int W = V == 1 ? 2 : 1;          // threads per line, in units of max(A, B)
+  static constexpr int ZT = cmin(64, W * cmax(A, B));
   static constexpr int LSA = M + 2;                 // natural-order line buffer (M + 1 bins)
   static constexpr int LSE = M + (M + A - 1) / A;   // exchange buffer, padded every A rows
-  static constexpr int LPB = cmax(1, pow2_floor(cmin((V + 1) * 256 / ZT, 4480 / (LSA + LSE))));
+  static constexpr int LPB0 = cmax(1, pow2_floor(cmin(W * 256 / ZT, 4480 / (LSA + LSE))));
+  static constexpr int LPB = V == 2 ? cmax(1, LPB0 / 2) : LPB0;
   static constexpr int THREADS = LPB * ZT;
 };
 
@@ -954,11 +957,12 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   }
 }
 
-// Waves per SIMD requested for k_zu: four (128 VGPRs) for the layout-0 kernel (the LDS
-// of its 4-wave block allows four per SIMD), except at the lengths where the compiler
-// then spills to scratch; else no constraint.
+// Waves per SIMD requested for k_zu: four (128 VGPRs) for layouts 0 and 2 (their LDS
+// allows four per SIMD), except at the lengths where the compiler then spills to
+// scratch; else no constraint.
 constexpr int zu_waves(int M, int ZV) {
-  return (ZV == 0 && M != 72 && M != 80 && M != 96 && M != 100 && M != 144 && M != 200) ? 4 : 1;
+  return (ZV != 1 && M != 72 && M != 80 && M != 96 && M != 100 && M != 144 && M != 200 &&
+          !(ZV == 2 && M == 120)) ? 4 : 1;
 }
 
 // velocity update, one axis per block. The velocity state is kept as its z-spectrum

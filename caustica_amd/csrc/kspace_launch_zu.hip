@@ -19,6 +19,7 @@ bool launch_zu(int M, int variant, const StepArgs& a, const float2* tw, const fl
 #define CST_ZU(N)                                  \
   case N:                                          \
     if (variant == 1) go_zu<N, 1>(a, tw, rt, st);  \
+    else if (variant == 2) go_zu<N, 2>(a, tw, rt, st); \
     else go_zu<N, 0>(a, tw, rt, st);               \
     return true;
     CST_LENGTHS(CST_ZU)

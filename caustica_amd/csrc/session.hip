@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(256) k_diag_copy(const float4* __restrict__ sr
 
 extern "C" {
 
-int cst_version(void) { return 100; }
+int cst_version(void) { return 101; }
 
 int cst_last_error(char* buf, size_t n) { return copy_last_error(buf, n); }
 
@@ -712,8 +712,8 @@ int cst_member_set_raw(cst_session* s, int slot, const void* buf, size_t bytes, 
 
 int cst_session_set_variant(cst_session* s, int pass, int variant) {
   if (!s) return fail(CST_ERR_ARG, "null session");
-  if (pass < 0 || pass > 3 || variant < 0 || variant > 1)
-    return fail(CST_ERR_ARG, "pass must be 0..3 and variant 0..1");
+  if (pass < 0 || pass > 3 || variant < 0 || variant > (pass == 2 ? 2 : 1))
+    return fail(CST_ERR_ARG, "pass must be 0..3 and variant 0..1 (0..2 for the velocity pass)");
   s->variant[pass] = variant;
   return CST_OK;
 }

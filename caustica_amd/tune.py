@@ -1,6 +1,6 @@
 """Autotuner for the thread-layout variants of the k-space passes.
 
-Each tunable pass (0 y, 1 x, 2 zu, 3 zp) has two instantiated thread layouts. For a
+Each tunable pass (2 zu, 3 zp) has two or three instantiated thread layouts. For a
 padded shape, ``autotune`` adds a throwaway member (zero state, the caller's plan)
 to the session, times every variant of every pass over a few steps with the
 library's event timing (variants interleaved over ``rounds`` rounds, the best round of
@@ -18,6 +18,7 @@ from pathlib import Path
 from caustica_amd import _native as N
 
 PASSES = {2: "zu", 3: "zp"}
+VARIANTS = {2: (0, 1, 2), 3: (0, 1)}
 _memo: dict = {}
 
 
@@ -76,7 +77,7 @@ def autotune(sess, plan, steps: int = 6, use_cache: bool = True, rounds: int = 2
             best = {}
             # interleaved rounds, best per-launch time of each variant: robust to drift
             for _ in range(rounds):
-                for v in (0, 1):
+                for v in VARIANTS[p]:
                     sess.set_variant(p, v)
                     sess.advance([slot], 1)
                     sess.set_profiling(True)

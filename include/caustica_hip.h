@@ -149,8 +149,9 @@ int cst_member_state_size(cst_session* s, size_t* bytes);
 int cst_member_get_raw(cst_session* s, int slot, void* buf, size_t bytes, int64_t* n);
 int cst_member_set_raw(cst_session* s, int slot, const void* buf, size_t bytes, int64_t n);
 
-/* Select the thread-layout variant (0 or 1) of a tunable pass (0 y, 1 x, 2 zu, 3 zp) for
- * every later step of the session; the autotuner times both and keeps the faster. */
+/* Select the thread-layout variant of a tunable pass (2 zu: 0, 1 or 2; 3 zp: 0 or 1;
+ * passes 0 and 1 accept 0 or 1 and have one layout) for every later step of the session;
+ * the autotuner times the variants and keeps the fastest. */
 int cst_session_set_variant(cst_session* s, int pass, int variant);
 
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the session
