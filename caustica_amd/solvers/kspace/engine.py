@@ -24,6 +24,7 @@ from math import ceil, isfinite
 
 import numpy as np
 
+from caustica_amd import _native as N
 from caustica_amd.core.backend import CausticaWarning, get_backend
 from caustica_amd.io.checkpoint import (
     CheckpointSpec,
@@ -153,6 +154,9 @@ def _fingerprint(plan: MemberPlan, solver_name, grid, medium, source, spec, back
         "spec": spec.model_dump(),
         "source_sha1": arrays_sha1(source.indices, source.phases),
         "medium_sha1": arrays_sha1(medium.c, medium.rho, medium.alpha, medium.beta),
+        # the raw device state's layout is the library's: a checkpoint resumes only on
+        # the library version that wrote it
+        "device_state_version": int(N.lib().cst_version()) if backend_name == "hip" else None,
     }
 
 
