@@ -788,6 +788,36 @@ struct Div24 {
   }
 };
 
+// Element j of a thread in a z-pass block's chunk of LPB lines x M float2: chunk index
+// i = threadIdx.x + j THREADS, line l = i / M, position mm = i % M; i < nl M exactly
+// when l < nl. When THREADS is a multiple of M, mm is the same for every j and l steps
+// by THREADS / M; when M is a multiple of THREADS, l is a compile-time constant per j.
+// Then no element needs a division, and where l is wave-uniform (THREADS = M, or M a
+// multiple of THREADS) the line-dependent parts of every address are scalar. Other
+// shapes divide with Div24.
+template <int M, int ZV>
+struct ZElem {
+  static constexpr int T = ZPlan<M, ZV>::THREADS;
+  static constexpr bool ROWS = T % M == 0;
+  static constexpr bool SPLIT = !ROWS && M % T == 0;
+  __device__ __forceinline__ static int line(int j) {
+    if constexpr (ROWS) {
+      // a wave's 64 lanes share threadIdx.x / M when M is a multiple of 64: say so
+      if constexpr (M % 64 == 0) return j * (T / M) + __builtin_amdgcn_readfirstlane((int)threadIdx.x / M);
+      else return j * (T / M) + (int)threadIdx.x / M;
+    } else if constexpr (SPLIT) {
+      return j / (M / T);
+    } else {
+      return Div24<M, ZCount<M, ZV>::EPT * T>::div((int)threadIdx.x + j * T);
+    }
+  }
+  __device__ __forceinline__ static int pos(int j, int l) {
+    if constexpr (ROWS) return (int)threadIdx.x % M;
+    else if constexpr (SPLIT) return (j % (M / T)) * T + (int)threadIdx.x;
+    else return (int)threadIdx.x + j * T - l * M;
+  }
+};
+
 // Tables a block stages into LDS (twiddles, real-split factors, z sponge, per-line
 // sponge product and spectrum-row offsets). They are loaded into registers first and
 // written to LDS after the block's HBM loads are issued: a wait for a load also waits
@@ -855,20 +885,20 @@ template <int M, int ZV>
 __device__ __forceinline__ void z_spec_load(gptr<const float2> spec, long line0, int nl,
                                             const Shape& sh, float2 (&r)[ZCount<M, ZV>::EPT],
                                             float2& rn) {
+  using E = ZElem<M, ZV>;
   const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;  // block-uniform
   // row of line (x, y) = x ps + y ms = line ms + 16 x (ps = ny ms + 16): offsets from the
   // block's first row fit 32 bits and need no per-element 64-bit multiply
   const gptr<const float2> base = spec + ((long)line0 * sh.ms + 16L * x0);
 #pragma unroll
   for (int j = 0; j < ZCount<M, ZV>::EPT; ++j) {
-    const int i0 = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
-    const int i = i0 < nl * M ? i0 : nl * M - 1;  // unconditional load (see z_stage_load)
-    const int l = Div24<M, ZPlan<M, ZV>::LPB * M>::div(i), k = i - l * M;
+    const int l0 = E::line(j), k = E::pos(j, l0);
+    const int l = l0 < nl ? l0 : nl - 1;  // unconditional load (see z_stage_load)
     int x, y;
     line_xy<ZPlan<M, ZV>::LPB>(x0, y0, l, sh, x, y);
     // 24-bit multiply and unsigned sums: the offset stays a full-rate 32-bit computation
-    const unsigned off = __umul24((unsigned)l, (unsigned)sh.ms) + 16u * (unsigned)(x - x0) + (unsigned)k;
-    r[j] = base[off];
+    const gptr<const float2> row = base + (__umul24((unsigned)l, (unsigned)sh.ms) + 16u * (unsigned)(x - x0));
+    r[j] = row[k];
   }
   rn = spec[sh.so + line0 + ((int)threadIdx.x < nl ? (int)threadIdx.x : nl - 1)];
 }
@@ -876,13 +906,11 @@ __device__ __forceinline__ void z_spec_load(gptr<const float2> spec, long line0,
 template <int M, int ZV>
 __device__ __forceinline__ void z_spec_put(int nl, const float2 (&r)[ZCount<M, ZV>::EPT], float2 rn,
                                            ZLds<M, ZV>& s) {
+  using E = ZElem<M, ZV>;
 #pragma unroll
   for (int j = 0; j < ZCount<M, ZV>::EPT; ++j) {
-    const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
-    if (i < nl * M) {
-      const int l = Div24<M, ZPlan<M, ZV>::LPB * M>::div(i), k = i - l * M;
-      s.a[l * ZPlan<M, ZV>::LSA + k] = r[j];
-    }
+    const int l = E::line(j), k = E::pos(j, l);
+    if (l < nl) s.a[l * ZPlan<M, ZV>::LSA + k] = r[j];
   }
   if ((int)threadIdx.x < nl) s.a[threadIdx.x * ZPlan<M, ZV>::LSA + M] = rn;
 }
@@ -891,10 +919,12 @@ __device__ __forceinline__ void z_spec_put(int nl, const float2 (&r)[ZCount<M, Z
 template <int M, int ZV>
 __device__ __forceinline__ void z_vol_load(gptr<const float2> v, int nl,
                                            float2 (&r)[ZCount<M, ZV>::EPT]) {
+  using E = ZElem<M, ZV>;
 #pragma unroll
   for (int j = 0; j < ZCount<M, ZV>::EPT; ++j) {
-    const int i = threadIdx.x + j * ZPlan<M, ZV>::THREADS;
-    r[j] = ld(v, i < nl * M ? i : nl * M - 1);  // unconditional load (see z_stage_load)
+    const int l0 = E::line(j), mm = E::pos(j, l0);
+    const int l = l0 < nl ? l0 : nl - 1;  // unconditional load (see z_stage_load)
+    r[j] = (v + l * M)[mm];
   }
 }
 
@@ -1017,9 +1047,8 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   float2 gg[CN::EPT];
 #pragma unroll
   for (int j = 0; j < CN::EPT; ++j) {
-    const int i = threadIdx.x + j * Z::THREADS;
-    const int ic = i < nl * M ? i : nl * M - 1;
-    const int l = Div24<M, Z::LPB * M>::div(ic), mm = ic - l * M;
+    const int l0 = ZElem<M, ZV>::line(j), mm = ZElem<M, ZV>::pos(j, l0);
+    const int l = l0 < nl ? l0 : nl - 1;
     gg[j] = s.a[l * Z::LSA + mm];
   }
   __syncthreads();
@@ -1028,9 +1057,8 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   z_c2r<M, ZV>(ll, t, lvalid, s);
 #pragma unroll
   for (int j = 0; j < CN::EPT; ++j) {
-    const int i = threadIdx.x + j * Z::THREADS;
-    if (i < nl * M) {
-      const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
+    const int l = ZElem<M, ZV>::line(j), mm = ZElem<M, ZV>::pos(j, l);
+    if (l < nl) {
       const float sxy = s.sxy[l];
       const float2 s2 = s.sz[mm];
       float2 uv = s.a[l * Z::LSA + mm];
@@ -1075,11 +1103,8 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
-      const int i = threadIdx.x + j * Z::THREADS;
-      if (i < nl * M) {
-        const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
-        s.a[l * Z::LSA + mm] = vv[j];
-      }
+      const int l = ZElem<M, ZV>::line(j), mm = ZElem<M, ZV>::pos(j, l);
+      if (l < nl) s.a[l * Z::LSA + mm] = vv[j];
     }
     __syncthreads();
     z_r2c<M, ZV>(ll, t, lvalid, s, uspec + s.row[lvalid ? ll : 0], uspec + a.s.so + line0 + (lvalid ? ll : 0));
@@ -1093,11 +1118,10 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     z_c2r<M, ZV>(ll, t, lvalid, s);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
-      const int i = threadIdx.x + j * Z::THREADS;
-      if (i < nl * M) {
-        const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
+      const int l = ZElem<M, ZV>::line(j), mm = ZElem<M, ZV>::pos(j, l);
+      if (l < nl) {
         const float2 v = s.a[l * Z::LSA + mm];
-        real[i] = make_float2(v.x * inz, v.y * inz);
+        (real + l * M)[mm] = make_float2(v.x * inz, v.y * inz);
       }
     }
   }
@@ -1229,9 +1253,9 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     };
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
-      const int i = threadIdx.x + j * Z::THREADS;
-      if (i < nl * M) {
-        const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
+      const int l = ZElem<M, ZV>::line(j), mm = ZElem<M, ZV>::pos(j, l);
+      if (l < nl) {
+        const int i = l * M + mm;
         const float sxy = s.sxy[l];
         const float2 dv = s.a[l * Z::LSA + mm];
         float2 pv = pp[j];
@@ -1296,11 +1320,8 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
-      const int i = threadIdx.x + j * Z::THREADS;
-      if (i < nl * M) {
-        const int l = Div24<M, Z::LPB * M>::div(i), mm = i - l * M;
-        s.a[l * Z::LSA + mm] = pp[j];
-      }
+      const int l = ZElem<M, ZV>::line(j), mm = ZElem<M, ZV>::pos(j, l);
+      if (l < nl) s.a[l * Z::LSA + mm] = pp[j];
     }
   }
   __syncthreads();

@@ -79,6 +79,8 @@ def main() -> int:
     ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--variant", type=int, default=0, help="z-pass thread layout (0 or 1)")
+    ap.add_argument("--zu-variant", type=int, help="velocity-pass layout (0, 1 or 2; default --variant)")
+    ap.add_argument("--zp-variant", type=int, help="pressure-pass layout (0 or 1; default --variant)")
     ap.add_argument("--record-steps", type=int, default=5, help="trailing steps that record")
     ap.add_argument("--summarize-stats", metavar="DIR")
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "WORKLOAD"))
@@ -102,13 +104,15 @@ def main() -> int:
     plan = plan_member(wl.grid, wl.medium, wl.source, wl.spec, harmonics=wl.harmonics,
                        reference_point=wl.reference_point, nonlinear=wl.solver == "westervelt")
     sess = KSpaceSession(plan.padded3, 1)
-    for p in (2, 3):
-        sess.set_variant(p, args.variant)
+    zu = args.variant if args.zu_variant is None else args.zu_variant
+    zp = args.variant if args.zp_variant is None else args.zp_variant
+    sess.set_variant(2, zu)
+    sess.set_variant(3, zp)
     slot = sess.add(plan, max_periods=args.steps // plan.spp + 4)
     if args.record_steps > 0:
         sess.schedule(slot, max(args.steps - args.record_steps, 0))
     print(f"[probe] {wl.name}: {args.steps} steps ({args.record_steps} recording), "
-          f"variant {args.variant}", flush=True)
+          f"layouts zu {zu} zp {zp}", flush=True)
     sess.advance([slot], args.steps)
     sess.sync()
     sess.close()
