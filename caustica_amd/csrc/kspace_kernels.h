@@ -72,6 +72,10 @@ struct ZPlan {
   static constexpr int LPB0 = cmax(1, pow2_floor(cmin(W * 256 / ZT, 4480 / (LSA + LSE))));
   static constexpr int LPB = V == 2 ? cmax(1, LPB0 / 2) : LPB0;
   static constexpr int THREADS = LPB * ZT;
+  // V = 2 with one-wave blocks: the natural and exchange line buffers share one LDS
+  // array (two more barriers per transform, free for a single wave), which lets more
+  // blocks share a CU's LDS where LDS rather than registers bounds the waves per SIMD
+  static constexpr bool ALIAS = V == 2 && THREADS <= 64;
 };
 
 template <int DIR>
@@ -731,8 +735,15 @@ __device__ __forceinline__ float2 r2c_post(const float2* l, int k, const float2*
 
 template <int M, int ZV>
 struct ZLds {
-  float2 a[ZPlan<M, ZV>::LPB * ZPlan<M, ZV>::LSA];
-  float2 e[ZPlan<M, ZV>::LPB * ZPlan<M, ZV>::LSE];
+  static constexpr int NA = ZPlan<M, ZV>::LPB * ZPlan<M, ZV>::LSA;
+  static constexpr int NE = ZPlan<M, ZV>::LPB * ZPlan<M, ZV>::LSE;
+  static constexpr bool ALIAS = ZPlan<M, ZV>::ALIAS;
+  float2 a[ALIAS ? cmax(NA, NE) : NA];
+  float2 ex[ALIAS ? 1 : NE];
+  __device__ __forceinline__ float2* e() {
+    if constexpr (ALIAS) return a;
+    else return ex;
+  }
   float2 tw[M];
   float2 rt[M + 1];
   float2 sz[M];  // z sponge profile as pairs (z = 2 mm, 2 mm + 1)
@@ -934,15 +945,16 @@ __device__ __forceinline__ void z_c2r(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   using Z = ZPlan<M, ZV>;
   using S = Split<M, Z::ZT>;
   float2* la = s.a + ll * Z::LSA;
-  float2* le = s.e + ll * Z::LSE;
+  float2* le = s.e() + ll * Z::LSE;
   float2 V[S::R1][S::B];
   in_load<M, Z::ZT>(t, [&](int n) { return c2r_pre<M>(la, n, s.rt); }, V);
   __syncthreads();  // a consumed
   if (lvalid) in_compute<M, Z::ZT, +1, true>(t, V, LinePadAt<S::A>{le}, s.tw);
   __syncthreads();
   float2 X[S::R2][S::A];
+  if (lvalid) in_finish<M, Z::ZT, +1>(t, LinePadAt<S::A>{le}, X);
+  if constexpr (Z::ALIAS) __syncthreads();  // e consumed before a (the same array) is written
   if (lvalid) {
-    in_finish<M, Z::ZT, +1>(t, LinePadAt<S::A>{le}, X);
 #pragma unroll
     for (int r = 0; r < S::R2; ++r) {
       const int k2 = t + Z::ZT * r;
@@ -963,14 +975,16 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   using Z = ZPlan<M, ZV>;
   using S = Split<M, Z::ZT>;
   float2* la = s.a + ll * Z::LSA;
-  float2* le = s.e + ll * Z::LSE;
+  float2* le = s.e() + ll * Z::LSE;
   float2 V[S::R1][S::B];
   in_load<M, Z::ZT>(t, [&](int n) { return la[n]; }, V);
+  if constexpr (Z::ALIAS) __syncthreads();  // a consumed before e (the same array) is written
   if (lvalid) in_compute<M, Z::ZT, -1, true>(t, V, LinePadAt<S::A>{le}, s.tw);
   __syncthreads();  // e written, a consumed
+  float2 X[S::R2][S::A];
+  if (lvalid) in_finish<M, Z::ZT, -1>(t, LinePadAt<S::A>{le}, X);
+  if constexpr (Z::ALIAS) __syncthreads();  // e consumed before a is written
   if (lvalid) {
-    float2 X[S::R2][S::A];
-    in_finish<M, Z::ZT, -1>(t, LinePadAt<S::A>{le}, X);
 #pragma unroll
     for (int r = 0; r < S::R2; ++r) {
       const int k2 = t + Z::ZT * r;
@@ -989,10 +1003,11 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
 
 // Waves per SIMD requested for k_zu: four (128 VGPRs) for layouts 0 and 2 (their LDS
 // allows four per SIMD), except at the lengths where the compiler then spills to
-// scratch; else no constraint.
+// scratch (for layout 2 with one-wave blocks, whose shared line buffer leaves the LDS
+// room for four: 150, 160, 192, 288, 300); else no constraint.
 constexpr int zu_waves(int M, int ZV) {
   return (ZV != 1 && M != 72 && M != 80 && M != 96 && M != 100 && M != 144 && M != 200 &&
-          !(ZV == 2 && M == 120)) ? 4 : 1;
+          !(ZV == 2 && (M == 120 || M == 150 || M == 160 || M == 192 || M == 288 || M == 300))) ? 4 : 1;
 }
 
 // velocity update, one axis per block. The velocity state is kept as its z-spectrum
