@@ -51,7 +51,7 @@ struct ThermalArgs {
 
 // Harmonic mean 2 a b / (a + b) of two conductances with the hardware reciprocal
 // (1 ulp) in place of the IEEE division sequence (thirteen instructions, one of them
-// quarter rate): the six face means per voxel were a third of the kernel's vector
+// quarter rate): the six face means per voxel were two fifths of the kernel's vector
 // instructions. Symmetric bitwise in (a, b): 2 a is exact, so 2 a b and 2 b a round
 // alike, which lets a plane reuse the x face its predecessor computed.
 __device__ __forceinline__ float hmean(float a, float b) {
