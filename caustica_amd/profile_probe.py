@@ -8,8 +8,8 @@ plain copy kernel over a buffer of known size (``cst_diag_copy``: 256 MiB read,
 the counter value of that calibration kernel converts the counter's units into bytes,
 which scales the per-dispatch values of the k-space kernels into HBM bytes per launch.
 
-``--summarize-stats DIR`` and ``--summarize-traffic FETCH_DIR WRITE_DIR WORKLOAD`` turn
-rocprofv3 CSV output (``--output-format csv -o run``) into the short text tables kept
+``--summarize-stats DIR``, ``--summarize-traffic FETCH_DIR WRITE_DIR WORKLOAD`` and
+``--summarize-sq A_DIR B_DIR`` turn rocprofv3 CSV output (``--output-format csv -o run``) into the short text tables kept
 with the measurements; the traffic table ends with ``traffic_bytes_per_launch`` lines
 (pass, workload, bytes) that bench.py reads.
 """
@@ -74,6 +74,50 @@ def summarize_traffic(fetch_dir: str, write_dir: str, workload: str) -> str:
     return "\n".join(out + [""] + lines)
 
 
+# MI355X: 256 CUs in 8 XCDs, 4 SIMDs per CU; a wave64 vector instruction occupies its
+# SIMD for 4 cycles
+N_CU, N_XCD, SIMD_CYCLES = 256, 8, 4
+
+
+def summarize_sq(a_dir: str, b_dir: str) -> str:
+    """Per-kernel shader-sequencer counters from two --pmc passes (SQ_* and GRBM_GUI_ACTIVE).
+
+    rocprofv3 sums each counter over the 8 XCDs; GRBM_GUI_ACTIVE / 8 is the kernel's
+    duration in cycles. VALU busy is the derived metric of the same name
+    (SQ_ACTIVE_INST_VALU per CU over that duration); "VALU issue" is the share of the
+    duration the kernel's vector instructions need at one wave64 instruction per SIMD every
+    4 cycles; waves per CU is 4 SQ_WAVE_CYCLES (counted in quad cycles) per CU over the
+    duration.
+    """
+    import csv
+    from collections import defaultdict
+
+    def load(d):
+        agg = defaultdict(lambda: defaultdict(list))
+        for r in csv.DictReader(open(f"{d}/run_counter_collection.csv")):
+            agg[_short(r["Kernel_Name"])][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        return {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in agg.items()}
+
+    A, B = load(a_dir), load(b_dir)
+    out = ["| kernel | waves | VALU instr / wave | VALU issue % | VALU busy % | waves / CU | "
+           "LDS instr / wave | LDS bank-conflict cycles % of LDS active |",
+           "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    for k, a in A.items():
+        if not k.startswith("cst::"):
+            continue
+        b = B.get(k, {})
+        cyc = a["GRBM_GUI_ACTIVE"] / N_XCD
+        waves = a["SQ_WAVES"]
+        vi = b.get("SQ_INSTS_VALU", 0.0)
+        issue = 100 * vi * SIMD_CYCLES / (4 * N_CU) / cyc
+        busy = 100 * a["SQ_ACTIVE_INST_VALU"] / N_CU / cyc
+        occ = 4 * a["SQ_WAVE_CYCLES"] / N_CU / cyc
+        conf = 100 * a["SQ_LDS_BANK_CONFLICT"] / max(a["SQ_ACTIVE_INST_LDS"], 1.0)
+        out.append(f"| `{k}` | {waves:.0f} | {vi / waves:.0f} | {issue:.0f} | {busy:.0f} | {occ:.1f} | "
+                   f"{a['SQ_INSTS_LDS'] / waves:.0f} | {conf:.0f} |")
+    return "\n".join(out)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
@@ -84,7 +128,11 @@ def main() -> int:
     ap.add_argument("--record-steps", type=int, default=5, help="trailing steps that record")
     ap.add_argument("--summarize-stats", metavar="DIR")
     ap.add_argument("--summarize-traffic", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "WORKLOAD"))
+    ap.add_argument("--summarize-sq", nargs=2, metavar=("A_DIR", "B_DIR"))
     args = ap.parse_args()
+    if args.summarize_sq:
+        print(summarize_sq(*args.summarize_sq))
+        return 0
     if args.summarize_stats:
         print(summarize_stats(args.summarize_stats))
         return 0
