@@ -3,7 +3,8 @@
 The variants (zu: 0, 1, 2; zp: 0, 1) change how lines are spread over threads and
 blocks, and so which products the compiler fuses into FMAs: their results differ at the
 float32 rounding level (2-3e-7 of the field per step). Checked on nz = 256 and 384 (the
-z lengths of the benchmark workloads) with small x/y, heterogeneous, absorbing and
+z lengths of the benchmark workloads) and 320 (a 16 x 10 split; like 384, one-wave blocks
+whose line buffers share LDS in layout 2) with small x/y, heterogeneous, absorbing and
 nonlinear: one step from a random state under every combination against the float64
 oracle (the one-step bar of tests/test_gpu_kspace.py); twenty more steps against the
 (0, 0) combination (within 1e-5 of the field); and the same 21 steps repeated from the
@@ -25,7 +26,7 @@ def _rel(a, b) -> float:
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-@pytest.mark.parametrize("shape", [(16, 24, 256), (8, 16, 384)])
+@pytest.mark.parametrize("shape", [(16, 24, 256), (8, 16, 384), (8, 8, 320)])
 def test_variants_agree_and_match_oracle(shape):
     from caustica_amd.solvers.kspace.native import KSpaceSession
     from caustica_amd.solvers.kspace.operators import plan_member
