@@ -75,7 +75,7 @@ __device__ __forceinline__ float cem43_rate(float t) {
 // barrier per plane.
 struct PennesPre {
   float tp, kp;            // centre theta, k of plane x+1 (for plane x)
-  float hty, hky, htz, hkz;  // halo theta, k of plane x
+  float ht, hk;            // theta, k of this thread's halo cell of plane x
   float q, pf, cf, tx;     // heat source, perfusion, dt/(rho C), theta_max of plane x
   double dz;               // dose of plane x
 };
@@ -95,16 +95,34 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
   const long plane = (long)a.ny * a.nz;
   const int zc = min(z, a.nz - 1), yc = min(y, a.ny - 1);
   const long col = (long)yc * a.nz + zc;
-  // halo roles: the first / last row of the tile loads the row above / below, the first /
-  // last column the cell before / after (clamped at the grid faces: unused there)
-  const bool hy = ty == 0 || ty == TY - 1;
-  const bool hz = tz == 0 || tz == TZ - 1;
-  const int yh = ty == 0 ? max(yc - 1, 0) : min(yc + 1, a.ny - 1);
-  const int zh = tz == 0 ? max(zc - 1, 0) : min(zc + 1, a.nz - 1);
-  const long colhy = (long)(hy ? yh : yc) * a.nz + zc;
-  const long colhz = (long)yc * a.nz + (hz ? zh : zc);
-  const int sy_h = ty == 0 ? 0 : TY + 1;
-  const int sz_h = tz == 0 ? 0 : TZ + 1;
+  // halo: 2 TZ cells of the rows above and below the tile, 2 TY cells before and after
+  // it in z. Thread i < 2 TZ loads a row cell (coalesced 256-byte rows), the next 2 TY
+  // threads a z cell, the rest none: one load pair per thread instead of a y and a z pair
+  // that most lanes would spend re-reading their own cell (clamped at the grid faces:
+  // unused there; lanes without a cell read their own and store nothing)
+  static_assert(TY * TZ >= 2 * TZ + 2 * TY, "one halo cell per thread at most");
+  const int y0 = blockIdx.y * TY, z0 = blockIdx.x * TZ;
+  const int hi = threadIdx.x;
+  const bool hrow = hi < 2 * TZ, hzc = !hrow && hi < 2 * TZ + 2 * TY;
+  const bool hv = hrow || hzc;
+  int hly, hlz;  // the cell's place in the LDS tile
+  long colh;
+  if (hrow) {
+    const int side = hi / TZ, zz = min(z0 + hi % TZ, a.nz - 1);
+    const int ye = min(y0 + (side ? TY - 1 : 0), a.ny - 1);
+    const int yy = side ? min(ye + 1, a.ny - 1) : max(ye - 1, 0);
+    hly = side ? TY + 1 : 0;
+    hlz = hi % TZ + 1;
+    colh = (long)yy * a.nz + zz;
+  } else {
+    const int j = hzc ? hi - 2 * TZ : 0, row = j >> 1, side = j & 1;
+    const int yy = min(y0 + row, a.ny - 1);
+    const int ze = min(z0 + (side ? TZ - 1 : 0), a.nz - 1);
+    const int zz = side ? min(ze + 1, a.nz - 1) : max(ze - 1, 0);
+    hly = row + 1;
+    hlz = side ? TZ + 1 : 0;
+    colh = hzc ? (long)yy * a.nz + zz : col;
+  }
   const int xlast = a.nx - 1;
   auto th = [&](int x, long c) { return a.th_in[(long)x * plane + c]; };
   auto kk = [&](int x, long c) { return a.k[(long)x * plane + c]; };
@@ -114,10 +132,8 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
     const long i = (long)xc * plane + col;
     p.tp = th(xp, col);
     p.kp = kk(xp, col);
-    p.hty = th(xc, colhy);
-    p.hky = kk(xc, colhy);
-    p.htz = th(xc, colhz);
-    p.hkz = kk(xc, colhz);
+    p.ht = th(xc, colh);
+    p.hk = kk(xc, colh);
     p.q = HAS_Q ? a.q[i] : 0.f;
     p.pf = a.perf[i];
     p.cf = a.coeff[i];
@@ -136,13 +152,9 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
     prefetch(x + 1, n);
     sth[b][ty + 1][tz + 1] = tc;
     sk[b][ty + 1][tz + 1] = kc;
-    if (hy) {
-      sth[b][sy_h][tz + 1] = c.hty;
-      sk[b][sy_h][tz + 1] = c.hky;
-    }
-    if (hz) {
-      sth[b][ty + 1][sz_h] = c.htz;
-      sk[b][ty + 1][sz_h] = c.hkz;
+    if (hv) {
+      sth[b][hly][hlz] = c.ht;
+      sk[b][hly][hlz] = c.hk;
     }
     __syncthreads();
     const float tp = c.tp, kp = c.kp;
