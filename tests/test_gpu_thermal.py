@@ -56,3 +56,30 @@ def test_pennes_finite_guard_raises():
     with pytest.raises(ThermalDivergedError):
         PennesSolver("hip", finite_check_every=1).solve(np.full(shape, 37.0), q, med, dt=0.1,
                                                         n_steps=3, dose=True)
+
+
+@pytest.mark.parametrize("shape", [(37, 13, 70), (64, 6, 130)])
+@pytest.mark.parametrize("boundary", ["insulated", "dirichlet"])
+def test_pennes_ragged_tiles_match_oracle(shape, boundary):
+    """Shapes that cut the kernel's 64 x 4 (z, y) tiles and 32-plane x chunks: a partial
+    last tile in z and y and a partial last chunk in x (37), and whole x chunks (64) with
+    a y extent below one tile row pair and z crossing three tiles. Heterogeneous
+    conductivity, perfusion and heat source, so every halo value enters the result."""
+    from caustica_amd.thermal import PennesSolver, ThermalMedium
+
+    print(f"[pennes] ragged {shape} {boundary}", flush=True)
+    rng = np.random.default_rng(sum(shape))
+    k = (0.2 + 0.3 * rng.random(shape)).astype(np.float32)
+    rho = (1000.0 + 80.0 * rng.random(shape)).astype(np.float32)
+    cp = (3500.0 + 300.0 * rng.random(shape)).astype(np.float32)
+    perf = (5e-4 * rng.random(shape)).astype(np.float32)
+    q = (3e7 * rng.random(shape) ** 4).astype(np.float32)
+    t0 = (37.0 + 0.5 * rng.standard_normal(shape)).astype(np.float32)
+    dx, dt, n = 5e-4, 0.05, 25
+    med = ThermalMedium(k=k, rho=rho, specific_heat=cp, perfusion=perf, dx=dx)
+    dev = PennesSolver("hip", boundary=boundary).solve(t0, q, med, dt=dt, n_steps=n, dose=True)
+    ref = PennesOracle(k, rho, cp, perf, dx, boundary=boundary, precision="fp64").solve(t0, q, dt, n, dose=True)
+    assert float(ref.temperature.max()) > 38.0  # the source heats
+    assert np.abs(dev.temperature - ref.temperature).max() < 1e-3
+    assert np.abs(dev.temperature_max - ref.temperature_max).max() < 1e-3
+    assert np.abs(dev.dose_cem43 - ref.dose).max() <= 1e-3 * np.abs(ref.dose).max()
