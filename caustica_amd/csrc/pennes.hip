@@ -212,6 +212,227 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
   if (bad) atomicOr(a.bad, 1u);
 }
 
+// ---------------------------------------------------------------- two steps per sweep
+// theta n -> n+2 in one pass over memory: per voxel and two steps, theta is read and
+// written once, k, Q, perfusion and dt/(rho C) read once, theta_max and the dose read
+// and written once: 48 P bytes per two steps instead of 96. Each block computes step n+1
+// on its TZ x TY tile plus a one-cell ring (its neighbours' tiles recompute that ring
+// too), from theta^n and k on the tile plus two rings, and step n+2 on the tile one
+// plane behind. Both steps use the same arithmetic as k_pennes, in the same order:
+// theta, theta_max and the dose come out as two launches of k_pennes would give them.
+constexpr int RW1 = TZ + 2, RW2 = TZ + 4;          // row lengths with one and two rings
+constexpr int NR1 = 2 * RW1 + 2 * TY;              // cells of ring 1 (140)
+constexpr int NR2 = 2 * RW2 + 2 * (TY + 2);        // cells of ring 2 (148)
+static_assert(NR1 <= TZ * TY && NR2 <= TZ * TY, "one ring-1 and one ring-2 cell per thread at most");
+
+// (dy, dz) of cell i of ring r (1 or 2) relative to the tile's first cell: the rows above
+// and below first, then the two ends of each row in between
+__device__ __forceinline__ void ring_cell(int r, int i, int& dy, int& dz) {
+  const int w = TZ + 2 * r;
+  if (i < 2 * w) {
+    const int side = i / w;
+    dz = i - side * w - r;
+    dy = side ? TY - 1 + r : -r;
+  } else {
+    const int j = i - 2 * w;
+    dy = (j >> 1) - (r - 1);
+    dz = (j & 1) ? TZ - 1 + r : -r;
+  }
+}
+
+// one explicit step of a voxel at (x, y, z): the k_pennes update, neighbours given
+__device__ __forceinline__ float pennes_new_theta(const ThermalArgs& a, int x, int y, int z, float tc,
+                                                  float kc, float tm, float tp, float kfm, float kfp,
+                                                  float tym, float kym, float typ, float kyp, float tzm,
+                                                  float kzm, float tzp, float kzp, float q, float pf,
+                                                  float cf, float iw) {
+  float work = 0.f;
+  if (x > 0) work += kfm * iw * (tm - tc);
+  else if (a.dirichlet & 1) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (x + 1 < a.nx) work += kfp * iw * (tp - tc);
+  else if (a.dirichlet & 1) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (y > 0) work += hmean(kym, kc) * iw * (tym - tc);
+  else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (y + 1 < a.ny) work += hmean(kc, kyp) * iw * (typ - tc);
+  else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (z > 0) work += hmean(kzm, kc) * iw * (tzm - tc);
+  else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (z + 1 < a.nz) work += hmean(kc, kzp) * iw * (tzp - tc);
+  else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
+  work += q;
+  work -= pf * tc;
+  return tc + work * cf;
+}
+
+// x planes per block: 64 keeps the grid (1024 blocks at 256^3, four 4-wave blocks per CU at
+// its 93 VGPRs) to a single round, and the two recomputed chunk-border planes to 3 %
+constexpr int XCH2 = 64;
+
+template <bool HAS_Q, bool DOSE>
+__global__ void __launch_bounds__(TZ* TY) k_pennes2(ThermalArgs a) {
+  __shared__ float sT[TY + 4][RW2];      // theta^n of plane xa on the tile + 2 rings
+  __shared__ float sK[2][TY + 4][RW2];   // k of planes xa (b) and xa - 1 (1 - b)
+  __shared__ float sU[2][TY + 2][RW1];   // theta^n+1 of planes xa (b) and xa - 1 (1 - b)
+  const int tid = threadIdx.x, tz = tid % TZ, ty = tid / TZ;
+  const int y0 = blockIdx.y * TY, z0 = blockIdx.x * TZ;
+  const int x0 = blockIdx.z * XCH2, x1 = min(x0 + XCH2, a.nx);
+  const long plane = (long)a.ny * a.nz;
+  const float iw = a.inv_dx2;
+  // the thread's cells: slot 0 the tile cell, slot 1 a ring-1 cell, slot 2 a ring-2 cell
+  int cy[3], cz[3];
+  cy[0] = ty;
+  cz[0] = tz;
+  const bool has1 = tid < NR1, has2 = tid < NR2;
+  ring_cell(1, has1 ? tid : 0, cy[1], cz[1]);
+  ring_cell(2, has2 ? tid : 0, cy[2], cz[2]);
+  bool in[3];
+  long col[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int y = y0 + cy[c], z = z0 + cz[c];
+    in[c] = y >= 0 && y < a.ny && z >= 0 && z < a.nz;
+    col[c] = (long)min(max(y, 0), a.ny - 1) * a.nz + min(max(z, 0), a.nz - 1);
+  }
+  in[1] = in[1] && has1;
+  in[2] = in[2] && has2;
+  const int xlast = a.nx - 1;
+  auto T = [&](int x, long c) { return a.th_in[(long)min(max(x, 0), xlast) * plane + c]; };
+  auto K = [&](int x, long c) { return a.k[(long)min(max(x, 0), xlast) * plane + c]; };
+  auto F = [&](const float* m, int x, long c) { return m[(long)min(max(x, 0), xlast) * plane + c]; };
+  // step-n registers of slots 0 and 1: theta and k of planes xa - 1, xa, xa + 1, and the
+  // x faces below and above xa
+  float tm[2], tc[2], tp[2], kc[2], kp[2], kfm[2];
+  const int xs = x0 - 1;  // first plane of step n+1
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    tm[c] = T(xs - 1, col[c]);
+    tc[c] = T(xs, col[c]);
+    tp[c] = T(xs + 1, col[c]);
+    const float km = K(xs - 1, col[c]);
+    kc[c] = K(xs, col[c]);
+    kp[c] = K(xs + 1, col[c]);
+    kfm[c] = hmean(km, kc[c]);
+  }
+  float t2 = T(xs, col[2]), k2 = K(xs, col[2]);  // slot 2: plane xa only
+  float mq[2], mpf[2], mcf[2];                    // maps of slots 0 and 1 at plane xa
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    mq[c] = HAS_Q ? F(a.q, xs, col[c]) : 0.f;
+    mpf[c] = F(a.perf, xs, col[c]);
+    mcf[c] = F(a.coeff, xs, col[c]);
+  }
+  // step n+2 state of the tile cell (plane xb = xa - 1): its theta^n, theta^n+1 of
+  // planes xb - 1, xb, k faces, maps, theta_max and dose
+  float u_m = 0.f, u_c = 0.f;          // theta^n+1 of the tile cell at xb - 1, xb
+  float tn_b = 0.f, kc_b = 0.f;        // theta^n, k of the tile cell at xb
+  float fm_b = 0.f, fp_b = 0.f;        // x faces below / above xb
+  float q_b = 0.f, pf_b = 0.f, cf_b = 0.f;
+  float tx_b = 0.f;    // set at the end of the first plane, before its first use
+  double dz_b = 0.0;
+  bool bad = false;
+  int b = 0;
+  for (int xa = xs; xa <= x1; ++xa) {
+    // plane xa (step n+1) and xb = xa - 1 (step n+2)
+    const int xb = xa - 1;
+    // prefetch: theta, k of slots 0, 1 at xa + 2 and slot 2 at xa + 1; maps of slots 0, 1
+    // at xa + 1; theta_max and dose of the tile cell at xa (step n+2 there next)
+    float nt[2], nk[2], nq[2], npf[2], ncf[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      nt[c] = T(xa + 2, col[c]);
+      nk[c] = K(xa + 2, col[c]);
+      nq[c] = HAS_Q ? F(a.q, xa + 1, col[c]) : 0.f;
+      npf[c] = F(a.perf, xa + 1, col[c]);
+      ncf[c] = F(a.coeff, xa + 1, col[c]);
+    }
+    const float nt2 = T(xa + 1, col[2]), nk2 = K(xa + 1, col[2]);
+    const float ntx = F(a.tmax, xa, col[0]);
+    const double ndz = DOSE ? a.dose[(long)min(max(xa, 0), xlast) * plane + col[0]] : 0.0;
+    // theta^n and k of plane xa on the tile and both rings
+    sT[cy[0] + 2][cz[0] + 2] = tc[0];
+    sK[b][cy[0] + 2][cz[0] + 2] = kc[0];
+    if (has1) {
+      sT[cy[1] + 2][cz[1] + 2] = tc[1];
+      sK[b][cy[1] + 2][cz[1] + 2] = kc[1];
+    }
+    if (has2) {
+      sT[cy[2] + 2][cz[2] + 2] = t2;
+      sK[b][cy[2] + 2][cz[2] + 2] = k2;
+    }
+    __syncthreads();
+    // step n+1 at plane xa for slots 0 and 1
+    const bool xa_in = xa >= 0 && xa < a.nx;
+    const float kfm0 = kfm[0];  // x face below xa of the tile cell
+    float u_new = 0.f, kfp0 = 0.f;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const float kfp = hmean(kc[c], kp[c]);
+      if (c == 0) kfp0 = kfp;
+      if (xa_in && in[c]) {
+        const int Y = cy[c] + 2, Z = cz[c] + 2;
+        const float un = pennes_new_theta(
+            a, xa, y0 + cy[c], z0 + cz[c], tc[c], kc[c], tm[c], tp[c], kfm[c], kfp, sT[Y - 1][Z],
+            sK[b][Y - 1][Z], sT[Y + 1][Z], sK[b][Y + 1][Z], sT[Y][Z - 1], sK[b][Y][Z - 1], sT[Y][Z + 1],
+            sK[b][Y][Z + 1], mq[c], mpf[c], mcf[c], iw);
+        sU[b][cy[c] + 1][cz[c] + 1] = un;
+        if (c == 0) u_new = un;
+      }
+      kfm[c] = kfp;
+    }
+    __syncthreads();
+    // step n+2 at plane xb for the tile cell (theta^n+1 of plane xb in sU[1 - b], of
+    // planes xb -+ 1 in u_m and u_new)
+    if (xb >= x0 && xb < x1 && in[0]) {
+      const int Y = cy[0] + 1, Z = cz[0] + 1, YK = cy[0] + 2, ZK = cz[0] + 2;
+      const float t2n = pennes_new_theta(
+          a, xb, y0 + cy[0], z0 + cz[0], u_c, kc_b, u_m, u_new, fm_b, fp_b, sU[1 - b][Y - 1][Z],
+          sK[1 - b][YK - 1][ZK], sU[1 - b][Y + 1][Z], sK[1 - b][YK + 1][ZK], sU[1 - b][Y][Z - 1],
+          sK[1 - b][YK][ZK - 1], sU[1 - b][Y][Z + 1], sK[1 - b][YK][ZK + 1], q_b, pf_b, cf_b, iw);
+      const long i = (long)xb * plane + col[0];
+      a.th_out[i] = t2n;
+      a.tmax[i] = fmaxf(fmaxf(tx_b, u_c), t2n);
+      bad |= !isfinite(u_c) || !isfinite(t2n);
+      if (DOSE) {
+        const float r0 = cem43_rate(tn_b + a.t_ref);
+        const float r1 = cem43_rate(u_c + a.t_ref);
+        const float r2 = cem43_rate(t2n + a.t_ref);
+        const double d1 = dz_b + ((double)r1 + (double)r0) * a.half_dt_min;
+        const double d2 = d1 + ((double)r2 + (double)r1) * a.half_dt_min;
+        a.dose[i] = d2;
+        bad |= !isfinite(d1) || !isfinite(d2);
+      }
+    }
+    __syncthreads();  // sT and sK[1 - b] are rewritten next
+    // rotate: the tile cell's plane-xa values become step n+2's plane xb
+    u_m = u_c;
+    u_c = u_new;
+    tn_b = tc[0];
+    kc_b = kc[0];
+    fm_b = kfm0;
+    fp_b = kfp0;
+    q_b = mq[0];
+    pf_b = mpf[0];
+    cf_b = mcf[0];
+    tx_b = ntx;
+    dz_b = ndz;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      tm[c] = tc[c];
+      tc[c] = tp[c];
+      tp[c] = nt[c];
+      kc[c] = kp[c];
+      kp[c] = nk[c];
+      mq[c] = nq[c];
+      mpf[c] = npf[c];
+      mcf[c] = ncf[c];
+    }
+    t2 = nt2;
+    k2 = nk2;
+    b = 1 - b;
+  }
+  if (bad) atomicOr(a.bad, 1u);
+}
+
 __global__ void k_dose_to_f32(const double* __restrict__ d, float* __restrict__ out, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (float)d[i];
@@ -325,7 +546,26 @@ int cst_thermal_run(cst_thermal* t, int n_substeps) {
   a.half_dt_min = t->d.half_dt_min;
   a.do_dose = t->d.dose;
   dim3 grid((t->nz + TZ - 1) / TZ, (t->ny + TY - 1) / TY, (t->nx + XCH - 1) / XCH);
-  for (int s = 0; s < n_substeps; ++s) {
+  int s = 0;
+  // pairs of sub-steps: two steps per sweep over memory (k_pennes2)
+  const dim3 grid2((t->nz + TZ - 1) / TZ, (t->ny + TY - 1) / TY, (t->nx + XCH2 - 1) / XCH2);
+  for (; s + 2 <= n_substeps; s += 2) {
+    a.th_in = t->theta[t->cur];
+    a.th_out = t->theta[1 - t->cur];
+    t->prof.run(0, t->stream, [&] {
+      const bool hq = a.q != nullptr, dd = a.do_dose != 0;
+      const dim3 bl(TZ * TY);
+      hipStream_t st = t->stream;
+      if (hq && dd) hipLaunchKernelGGL((k_pennes2<true, true>), grid2, bl, 0, st, a);
+      else if (hq) hipLaunchKernelGGL((k_pennes2<true, false>), grid2, bl, 0, st, a);
+      else if (dd) hipLaunchKernelGGL((k_pennes2<false, true>), grid2, bl, 0, st, a);
+      else hipLaunchKernelGGL((k_pennes2<false, false>), grid2, bl, 0, st, a);
+      return true;
+    });
+    HIPCHK(hipGetLastError());
+    t->cur = 1 - t->cur;
+  }
+  for (; s < n_substeps; ++s) {
     a.th_in = t->theta[t->cur];
     a.th_out = t->theta[1 - t->cur];
     t->prof.run(0, t->stream, [&] {
