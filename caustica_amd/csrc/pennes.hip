@@ -268,6 +268,16 @@ __device__ __forceinline__ float pennes_new_theta(const ThermalArgs& a, int x, i
 // its 93 VGPRs) to a single round, and the two recomputed chunk-border planes to 3 %
 constexpr int XCH2 = 64;
 
+// Loads of one plane step, issued one iteration before they are consumed: theta, k of
+// slots 0 and 1 at xa + 1, their maps at xa, theta, k of slot 2 at xa, theta_max and dose
+// of the tile cell at xa - 1 (for step n+2 there). Two sets alternate (the march is
+// unrolled twice), so consuming one never waits on the loads just issued into the other.
+struct Pennes2Pre {
+  float t[2], k[2], q[2], pf[2], cf[2];
+  float t2, k2, tx;
+  double dz;
+};
+
 template <bool HAS_Q, bool DOSE>
 __global__ void __launch_bounds__(TZ* TY) k_pennes2(ThermalArgs a) {
   __shared__ float sT[TY + 4][RW2];      // theta^n of plane xa on the tile + 2 rings
@@ -296,58 +306,41 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes2(ThermalArgs a) {
   in[1] = in[1] && has1;
   in[2] = in[2] && has2;
   const int xlast = a.nx - 1;
-  auto T = [&](int x, long c) { return a.th_in[(long)min(max(x, 0), xlast) * plane + c]; };
-  auto K = [&](int x, long c) { return a.k[(long)min(max(x, 0), xlast) * plane + c]; };
   auto F = [&](const float* m, int x, long c) { return m[(long)min(max(x, 0), xlast) * plane + c]; };
-  // step-n registers of slots 0 and 1: theta and k of planes xa - 1, xa, xa + 1, and the
-  // x faces below and above xa
-  float tm[2], tc[2], tp[2], kc[2], kp[2], kfm[2];
+  auto prefetch = [&](int xa, Pennes2Pre& p) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      p.t[c] = F(a.th_in, xa + 1, col[c]);
+      p.k[c] = F(a.k, xa + 1, col[c]);
+      p.q[c] = HAS_Q ? F(a.q, xa, col[c]) : 0.f;
+      p.pf[c] = F(a.perf, xa, col[c]);
+      p.cf[c] = F(a.coeff, xa, col[c]);
+    }
+    p.t2 = F(a.th_in, xa, col[2]);
+    p.k2 = F(a.k, xa, col[2]);
+    p.tx = F(a.tmax, xa - 1, col[0]);
+    p.dz = DOSE ? a.dose[(long)min(max(xa - 1, 0), xlast) * plane + col[0]] : 0.0;
+  };
+  // carried step-n state of slots 0 and 1: theta at xa - 1 and xa, k at xa, x face below xa
+  float tm[2], tc[2], kc[2], kfm[2];
   const int xs = x0 - 1;  // first plane of step n+1
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
-    tm[c] = T(xs - 1, col[c]);
-    tc[c] = T(xs, col[c]);
-    tp[c] = T(xs + 1, col[c]);
-    const float km = K(xs - 1, col[c]);
-    kc[c] = K(xs, col[c]);
-    kp[c] = K(xs + 1, col[c]);
-    kfm[c] = hmean(km, kc[c]);
+    tm[c] = F(a.th_in, xs - 1, col[c]);
+    tc[c] = F(a.th_in, xs, col[c]);
+    kc[c] = F(a.k, xs, col[c]);
+    kfm[c] = hmean(F(a.k, xs - 1, col[c]), kc[c]);
   }
-  float t2 = T(xs, col[2]), k2 = K(xs, col[2]);  // slot 2: plane xa only
-  float mq[2], mpf[2], mcf[2];                    // maps of slots 0 and 1 at plane xa
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    mq[c] = HAS_Q ? F(a.q, xs, col[c]) : 0.f;
-    mpf[c] = F(a.perf, xs, col[c]);
-    mcf[c] = F(a.coeff, xs, col[c]);
-  }
-  // step n+2 state of the tile cell (plane xb = xa - 1): its theta^n, theta^n+1 of
-  // planes xb - 1, xb, k faces, maps, theta_max and dose
-  float u_m = 0.f, u_c = 0.f;          // theta^n+1 of the tile cell at xb - 1, xb
-  float tn_b = 0.f, kc_b = 0.f;        // theta^n, k of the tile cell at xb
-  float fm_b = 0.f, fp_b = 0.f;        // x faces below / above xb
+  Pennes2Pre P0, P1;
+  prefetch(xs, P0);
+  // step n+2 state of the tile cell at xb = xa - 1: theta^n+1 at xb - 1 and xb, theta^n,
+  // k, x faces and maps at xb
+  float u_m = 0.f, u_c = 0.f, tn_b = 0.f, kc_b = 0.f, fm_b = 0.f, fp_b = 0.f;
   float q_b = 0.f, pf_b = 0.f, cf_b = 0.f;
-  float tx_b = 0.f;    // set at the end of the first plane, before its first use
-  double dz_b = 0.0;
   bool bad = false;
-  int b = 0;
-  for (int xa = xs; xa <= x1; ++xa) {
-    // plane xa (step n+1) and xb = xa - 1 (step n+2)
+  auto iter = [&](int xa, int b, const Pennes2Pre& cur, Pennes2Pre& nxt) {
     const int xb = xa - 1;
-    // prefetch: theta, k of slots 0, 1 at xa + 2 and slot 2 at xa + 1; maps of slots 0, 1
-    // at xa + 1; theta_max and dose of the tile cell at xa (step n+2 there next)
-    float nt[2], nk[2], nq[2], npf[2], ncf[2];
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      nt[c] = T(xa + 2, col[c]);
-      nk[c] = K(xa + 2, col[c]);
-      nq[c] = HAS_Q ? F(a.q, xa + 1, col[c]) : 0.f;
-      npf[c] = F(a.perf, xa + 1, col[c]);
-      ncf[c] = F(a.coeff, xa + 1, col[c]);
-    }
-    const float nt2 = T(xa + 1, col[2]), nk2 = K(xa + 1, col[2]);
-    const float ntx = F(a.tmax, xa, col[0]);
-    const double ndz = DOSE ? a.dose[(long)min(max(xa, 0), xlast) * plane + col[0]] : 0.0;
+    prefetch(xa + 1, nxt);
     // theta^n and k of plane xa on the tile and both rings
     sT[cy[0] + 2][cz[0] + 2] = tc[0];
     sK[b][cy[0] + 2][cz[0] + 2] = kc[0];
@@ -356,24 +349,24 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes2(ThermalArgs a) {
       sK[b][cy[1] + 2][cz[1] + 2] = kc[1];
     }
     if (has2) {
-      sT[cy[2] + 2][cz[2] + 2] = t2;
-      sK[b][cy[2] + 2][cz[2] + 2] = k2;
+      sT[cy[2] + 2][cz[2] + 2] = cur.t2;
+      sK[b][cy[2] + 2][cz[2] + 2] = cur.k2;
     }
     __syncthreads();
     // step n+1 at plane xa for slots 0 and 1
     const bool xa_in = xa >= 0 && xa < a.nx;
-    const float kfm0 = kfm[0];  // x face below xa of the tile cell
+    const float kfm0 = kfm[0];
     float u_new = 0.f, kfp0 = 0.f;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
-      const float kfp = hmean(kc[c], kp[c]);
+      const float kfp = hmean(kc[c], cur.k[c]);
       if (c == 0) kfp0 = kfp;
       if (xa_in && in[c]) {
         const int Y = cy[c] + 2, Z = cz[c] + 2;
         const float un = pennes_new_theta(
-            a, xa, y0 + cy[c], z0 + cz[c], tc[c], kc[c], tm[c], tp[c], kfm[c], kfp, sT[Y - 1][Z],
+            a, xa, y0 + cy[c], z0 + cz[c], tc[c], kc[c], tm[c], cur.t[c], kfm[c], kfp, sT[Y - 1][Z],
             sK[b][Y - 1][Z], sT[Y + 1][Z], sK[b][Y + 1][Z], sT[Y][Z - 1], sK[b][Y][Z - 1], sT[Y][Z + 1],
-            sK[b][Y][Z + 1], mq[c], mpf[c], mcf[c], iw);
+            sK[b][Y][Z + 1], cur.q[c], cur.pf[c], cur.cf[c], iw);
         sU[b][cy[c] + 1][cz[c] + 1] = un;
         if (c == 0) u_new = un;
       }
@@ -390,45 +383,39 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes2(ThermalArgs a) {
           sK[1 - b][YK][ZK - 1], sU[1 - b][Y][Z + 1], sK[1 - b][YK][ZK + 1], q_b, pf_b, cf_b, iw);
       const long i = (long)xb * plane + col[0];
       a.th_out[i] = t2n;
-      a.tmax[i] = fmaxf(fmaxf(tx_b, u_c), t2n);
+      a.tmax[i] = fmaxf(fmaxf(cur.tx, u_c), t2n);
       bad |= !isfinite(u_c) || !isfinite(t2n);
       if (DOSE) {
         const float r0 = cem43_rate(tn_b + a.t_ref);
         const float r1 = cem43_rate(u_c + a.t_ref);
         const float r2 = cem43_rate(t2n + a.t_ref);
-        const double d1 = dz_b + ((double)r1 + (double)r0) * a.half_dt_min;
+        const double d1 = cur.dz + ((double)r1 + (double)r0) * a.half_dt_min;
         const double d2 = d1 + ((double)r2 + (double)r1) * a.half_dt_min;
         a.dose[i] = d2;
         bad |= !isfinite(d1) || !isfinite(d2);
       }
     }
     __syncthreads();  // sT and sK[1 - b] are rewritten next
-    // rotate: the tile cell's plane-xa values become step n+2's plane xb
+    // the tile cell's plane-xa values become step n+2's plane xb next
     u_m = u_c;
     u_c = u_new;
     tn_b = tc[0];
     kc_b = kc[0];
     fm_b = kfm0;
     fp_b = kfp0;
-    q_b = mq[0];
-    pf_b = mpf[0];
-    cf_b = mcf[0];
-    tx_b = ntx;
-    dz_b = ndz;
+    q_b = cur.q[0];
+    pf_b = cur.pf[0];
+    cf_b = cur.cf[0];
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       tm[c] = tc[c];
-      tc[c] = tp[c];
-      tp[c] = nt[c];
-      kc[c] = kp[c];
-      kp[c] = nk[c];
-      mq[c] = nq[c];
-      mpf[c] = npf[c];
-      mcf[c] = ncf[c];
+      tc[c] = cur.t[c];
+      kc[c] = cur.k[c];
     }
-    t2 = nt2;
-    k2 = nk2;
-    b = 1 - b;
+  };
+  for (int xa = xs; xa <= x1; xa += 2) {
+    iter(xa, 0, P0, P1);
+    if (xa + 1 <= x1) iter(xa + 1, 1, P1, P0);
   }
   if (bad) atomicOr(a.bad, 1u);
 }
