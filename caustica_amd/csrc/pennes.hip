@@ -1,4 +1,5 @@
-// Pennes bioheat + CEM43 dose: one fused explicit step per launch.
+// Pennes bioheat + CEM43 dose: one fused explicit step per launch (k_pennes), or two
+// (k_pennes2).
 //
 // theta = T - T_ref (float32), per sub-step (upstream thermal/pennes.py:608-654):
 //   work  = sum_faces k_face / dx^2 (theta_nb - theta)   [k_face = 2 k_i k_j / (k_i + k_j)]
@@ -64,6 +65,31 @@ __device__ __forceinline__ float hmean(float a, float b) {
 __device__ __forceinline__ float cem43_rate(float t) {
   const float d = t - 43.f;
   return __builtin_amdgcn_exp2f(t > 43.f ? d : 2.f * d);
+}
+
+// one explicit step of a voxel at (x, y, z), its neighbours' theta and k given: the update
+// both kernels run
+__device__ __forceinline__ float pennes_new_theta(const ThermalArgs& a, int x, int y, int z, float tc,
+                                                  float kc, float tm, float tp, float kfm, float kfp,
+                                                  float tym, float kym, float typ, float kyp, float tzm,
+                                                  float kzm, float tzp, float kzp, float q, float pf,
+                                                  float cf, float iw) {
+  float work = 0.f;
+  if (x > 0) work += kfm * iw * (tm - tc);
+  else if (a.dirichlet & 1) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (x + 1 < a.nx) work += kfp * iw * (tp - tc);
+  else if (a.dirichlet & 1) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (y > 0) work += hmean(kym, kc) * iw * (tym - tc);
+  else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (y + 1 < a.ny) work += hmean(kc, kyp) * iw * (typ - tc);
+  else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (z > 0) work += hmean(kzm, kc) * iw * (tzm - tc);
+  else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
+  if (z + 1 < a.nz) work += hmean(kc, kzp) * iw * (tzp - tc);
+  else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
+  work += q;
+  work -= pf * tc;
+  return tc + work * cf;
 }
 
 // Software-pipelined march along x: while plane x is computed, the loads for plane x+1
@@ -161,25 +187,10 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
     const float kfp = hmean(kc, kp);  // x face above plane x
     if (valid) {
       const long i = (long)x * plane + col;
-      float work = 0.f;
-      // x faces
-      if (x > 0) work += kfm * iw * (tm - tc);
-      else if (a.dirichlet & 1) work += 2.f * kc * iw * (a.theta_wall - tc);
-      if (x + 1 < a.nx) work += kfp * iw * (tp - tc);
-      else if (a.dirichlet & 1) work += 2.f * kc * iw * (a.theta_wall - tc);
-      // y faces
-      if (y > 0) work += hmean(sk[b][ty][tz + 1], kc) * iw * (sth[b][ty][tz + 1] - tc);
-      else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
-      if (y + 1 < a.ny) work += hmean(kc, sk[b][ty + 2][tz + 1]) * iw * (sth[b][ty + 2][tz + 1] - tc);
-      else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
-      // z faces
-      if (z > 0) work += hmean(sk[b][ty + 1][tz], kc) * iw * (sth[b][ty + 1][tz] - tc);
-      else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
-      if (z + 1 < a.nz) work += hmean(kc, sk[b][ty + 1][tz + 2]) * iw * (sth[b][ty + 1][tz + 2] - tc);
-      else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
-      if (HAS_Q) work += c.q;
-      work -= c.pf * tc;
-      const float tn = tc + work * c.cf;
+      const float tn = pennes_new_theta(
+          a, x, y, z, tc, kc, tm, tp, kfm, kfp, sth[b][ty][tz + 1], sk[b][ty][tz + 1], sth[b][ty + 2][tz + 1],
+          sk[b][ty + 2][tz + 1], sth[b][ty + 1][tz], sk[b][ty + 1][tz], sth[b][ty + 1][tz + 2],
+          sk[b][ty + 1][tz + 2], HAS_Q ? c.q : 0.f, c.pf, c.cf, iw);
       a.th_out[i] = tn;
       a.tmax[i] = fmaxf(c.tx, tn);
       bad |= !isfinite(tn);
@@ -218,8 +229,8 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
 // and written once: 48 P bytes per two steps instead of 96. Each block computes step n+1
 // on its TZ x TY tile plus a one-cell ring (its neighbours' tiles recompute that ring
 // too), from theta^n and k on the tile plus two rings, and step n+2 on the tile one
-// plane behind. Both steps use the same arithmetic as k_pennes, in the same order:
-// theta, theta_max and the dose come out as two launches of k_pennes would give them.
+// plane behind. Both steps run k_pennes's update (pennes_new_theta) and take theta_max
+// and the dose in the same order: two steps come out as two launches of k_pennes give them.
 constexpr int RW1 = TZ + 2, RW2 = TZ + 4;          // row lengths with one and two rings
 constexpr int NR1 = 2 * RW1 + 2 * TY;              // cells of ring 1 (140)
 constexpr int NR2 = 2 * RW2 + 2 * (TY + 2);        // cells of ring 2 (148)
@@ -240,29 +251,6 @@ __device__ __forceinline__ void ring_cell(int r, int i, int& dy, int& dz) {
   }
 }
 
-// one explicit step of a voxel at (x, y, z): the k_pennes update, neighbours given
-__device__ __forceinline__ float pennes_new_theta(const ThermalArgs& a, int x, int y, int z, float tc,
-                                                  float kc, float tm, float tp, float kfm, float kfp,
-                                                  float tym, float kym, float typ, float kyp, float tzm,
-                                                  float kzm, float tzp, float kzp, float q, float pf,
-                                                  float cf, float iw) {
-  float work = 0.f;
-  if (x > 0) work += kfm * iw * (tm - tc);
-  else if (a.dirichlet & 1) work += 2.f * kc * iw * (a.theta_wall - tc);
-  if (x + 1 < a.nx) work += kfp * iw * (tp - tc);
-  else if (a.dirichlet & 1) work += 2.f * kc * iw * (a.theta_wall - tc);
-  if (y > 0) work += hmean(kym, kc) * iw * (tym - tc);
-  else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
-  if (y + 1 < a.ny) work += hmean(kc, kyp) * iw * (typ - tc);
-  else if (a.dirichlet & 2) work += 2.f * kc * iw * (a.theta_wall - tc);
-  if (z > 0) work += hmean(kzm, kc) * iw * (tzm - tc);
-  else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
-  if (z + 1 < a.nz) work += hmean(kc, kzp) * iw * (tzp - tc);
-  else if (a.dirichlet & 4) work += 2.f * kc * iw * (a.theta_wall - tc);
-  work += q;
-  work -= pf * tc;
-  return tc + work * cf;
-}
 
 // x planes per block: 64 keeps the grid (1024 blocks at 256^3, four 4-wave blocks per CU at
 // its 93 VGPRs) to a single round, and the two recomputed chunk-border planes to 3 %

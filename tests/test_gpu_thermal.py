@@ -83,3 +83,33 @@ def test_pennes_ragged_tiles_match_oracle(shape, boundary):
     assert np.abs(dev.temperature - ref.temperature).max() < 1e-3
     assert np.abs(dev.temperature_max - ref.temperature_max).max() < 1e-3
     assert np.abs(dev.dose_cem43 - ref.dose).max() <= 1e-3 * np.abs(ref.dose).max()
+
+
+@pytest.mark.parametrize("boundary", ["insulated", "dirichlet"])
+def test_pennes_two_step_launch_equals_two_one_step_launches(boundary):
+    """The two-step kernel against the one-step kernel launched twice. The host launches
+    single steps when the non-finite check comes every step and pairs when it comes every
+    second step; both kernels run the same update function and take theta_max and the
+    dose in the same order, so the results agree bit for bit. Ragged in every axis
+    (x = 70 cuts both kernels' x chunks)."""
+    from caustica_amd.thermal import PennesSolver, ThermalMedium
+
+    shape = (70, 13, 37)
+    print(f"[pennes] two-step vs one-step {shape} {boundary}", flush=True)
+    rng = np.random.default_rng(7)
+    k = (0.2 + 0.3 * rng.random(shape)).astype(np.float32)
+    rho = (1000.0 + 80.0 * rng.random(shape)).astype(np.float32)
+    cp = (3500.0 + 300.0 * rng.random(shape)).astype(np.float32)
+    perf = (5e-4 * rng.random(shape)).astype(np.float32)
+    q = (3e7 * rng.random(shape) ** 4).astype(np.float32)
+    t0 = (42.0 + 0.5 * rng.standard_normal(shape)).astype(np.float32)
+    med = ThermalMedium(k=k, rho=rho, specific_heat=cp, perfusion=perf, dx=5e-4)
+    out = []
+    for every in (1, 2):
+        s = PennesSolver("hip", boundary=boundary, finite_check_every=every)
+        out.append(s.solve(t0, q, med, dt=0.05, n_steps=8, dose=True))
+    one, two = out
+    for name in ("temperature", "temperature_max", "dose_cem43"):
+        a, b = getattr(one, name), getattr(two, name)
+        print(f"[pennes] {name}: max |diff| {float(np.abs(a - b).max()):.3e}", flush=True)
+        assert np.array_equal(a, b), name
