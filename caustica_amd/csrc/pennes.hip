@@ -227,14 +227,15 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
 // theta n -> n+2 in one pass over memory: per voxel and two steps, theta is read and
 // written once, k, Q, perfusion and dt/(rho C) read once, theta_max and the dose read
 // and written once: 48 P bytes per two steps instead of 96. Each block computes step n+1
-// on its TZ x TY tile plus a one-cell ring (its neighbours' tiles recompute that ring
+// on its TZ x TY2 tile plus a one-cell ring (its neighbours' tiles recompute that ring
 // too), from theta^n and k on the tile plus two rings, and step n+2 on the tile one
 // plane behind. Both steps run k_pennes's update (pennes_new_theta) and take theta_max
 // and the dose in the same order: two steps come out as two launches of k_pennes give them.
+constexpr int TY2 = 8;                             // tile rows (y): 64 x 8, eight waves
 constexpr int RW1 = TZ + 2, RW2 = TZ + 4;          // row lengths with one and two rings
-constexpr int NR1 = 2 * RW1 + 2 * TY;              // cells of ring 1 (140)
-constexpr int NR2 = 2 * RW2 + 2 * (TY + 2);        // cells of ring 2 (148)
-static_assert(NR1 <= TZ * TY && NR2 <= TZ * TY, "one ring-1 and one ring-2 cell per thread at most");
+constexpr int NR1 = 2 * RW1 + 2 * TY2;             // cells of ring 1 (148)
+constexpr int NR2 = 2 * RW2 + 2 * (TY2 + 2);       // cells of ring 2 (156)
+static_assert(NR1 <= TZ * TY2 && NR2 <= TZ * TY2, "one ring-1 and one ring-2 cell per thread at most");
 
 // (dy, dz) of cell i of ring r (1 or 2) relative to the tile's first cell: the rows above
 // and below first, then the two ends of each row in between
@@ -243,7 +244,7 @@ __device__ __forceinline__ void ring_cell(int r, int i, int& dy, int& dz) {
   if (i < 2 * w) {
     const int side = i / w;
     dz = i - side * w - r;
-    dy = side ? TY - 1 + r : -r;
+    dy = side ? TY2 - 1 + r : -r;
   } else {
     const int j = i - 2 * w;
     dy = (j >> 1) - (r - 1);
@@ -252,8 +253,9 @@ __device__ __forceinline__ void ring_cell(int r, int i, int& dy, int& dz) {
 }
 
 
-// x planes per block: 64 keeps the grid (1024 blocks at 256^3, four 4-wave blocks per CU at
-// its 93 VGPRs) to a single round, and the two recomputed chunk-border planes to 3 %
+// x planes per block: 64 keeps the grid (512 blocks at 256^3, two 8-wave blocks per CU) to
+// a single round, and the two recomputed chunk-border planes to 3 %. The 64 x 8 tile
+// keeps the recomputed ring at 29 % of the tile (55 % for 64 x 4)
 constexpr int XCH2 = 64;
 
 // Loads of one plane step, issued one iteration before they are consumed: theta, k of
@@ -267,12 +269,12 @@ struct Pennes2Pre {
 };
 
 template <bool HAS_Q, bool DOSE>
-__global__ void __launch_bounds__(TZ* TY) k_pennes2(ThermalArgs a) {
-  __shared__ float sT[TY + 4][RW2];      // theta^n of plane xa on the tile + 2 rings
-  __shared__ float sK[2][TY + 4][RW2];   // k of planes xa (b) and xa - 1 (1 - b)
-  __shared__ float sU[2][TY + 2][RW1];   // theta^n+1 of planes xa (b) and xa - 1 (1 - b)
+__global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
+  __shared__ float sT[TY2 + 4][RW2];      // theta^n of plane xa on the tile + 2 rings
+  __shared__ float sK[2][TY2 + 4][RW2];   // k of planes xa (b) and xa - 1 (1 - b)
+  __shared__ float sU[2][TY2 + 2][RW1];   // theta^n+1 of planes xa (b) and xa - 1 (1 - b)
   const int tid = threadIdx.x, tz = tid % TZ, ty = tid / TZ;
-  const int y0 = blockIdx.y * TY, z0 = blockIdx.x * TZ;
+  const int y0 = blockIdx.y * TY2, z0 = blockIdx.x * TZ;
   const int x0 = blockIdx.z * XCH2, x1 = min(x0 + XCH2, a.nx);
   const long plane = (long)a.ny * a.nz;
   const float iw = a.inv_dx2;
@@ -523,13 +525,13 @@ int cst_thermal_run(cst_thermal* t, int n_substeps) {
   dim3 grid((t->nz + TZ - 1) / TZ, (t->ny + TY - 1) / TY, (t->nx + XCH - 1) / XCH);
   int s = 0;
   // pairs of sub-steps: two steps per sweep over memory (k_pennes2)
-  const dim3 grid2((t->nz + TZ - 1) / TZ, (t->ny + TY - 1) / TY, (t->nx + XCH2 - 1) / XCH2);
+  const dim3 grid2((t->nz + TZ - 1) / TZ, (t->ny + TY2 - 1) / TY2, (t->nx + XCH2 - 1) / XCH2);
   for (; s + 2 <= n_substeps; s += 2) {
     a.th_in = t->theta[t->cur];
     a.th_out = t->theta[1 - t->cur];
     t->prof.run(0, t->stream, [&] {
       const bool hq = a.q != nullptr, dd = a.do_dose != 0;
-      const dim3 bl(TZ * TY);
+      const dim3 bl(TZ * TY2);
       hipStream_t st = t->stream;
       if (hq && dd) hipLaunchKernelGGL((k_pennes2<true, true>), grid2, bl, 0, st, a);
       else if (hq) hipLaunchKernelGGL((k_pennes2<true, false>), grid2, bl, 0, st, a);
