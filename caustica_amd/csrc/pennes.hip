@@ -48,6 +48,7 @@ struct ThermalArgs {
   float t_ref;
   double half_dt_min;
   int do_dose;
+  int xch2;  // x planes per block of k_pennes2
 };
 
 // Harmonic mean 2 a b / (a + b) of two conductances with the hardware reciprocal
@@ -253,10 +254,11 @@ __device__ __forceinline__ void ring_cell(int r, int i, int& dy, int& dz) {
 }
 
 
-// x planes per block: 64 keeps the grid (512 blocks at 256^3, two 8-wave blocks per CU) to
-// a single round, and the two recomputed chunk-border planes to 3 %. The 64 x 8 tile
-// keeps the recomputed ring at 29 % of the tile (55 % for 64 x 4)
-constexpr int XCH2 = 64;
+// x planes per block (a.xch2): the longest of 64, 32, 16, 8 that still gives the grid at
+// least two blocks per CU (at 256^3: 64, 512 blocks in one round, the two recomputed
+// chunk-border planes 3 % of the work; smaller grids take shorter chunks rather than
+// leave CUs idle). The 64 x 8 tile keeps the recomputed ring at 29 % of the tile (55 %
+// for 64 x 4).
 
 // Loads of one plane step, issued one iteration before they are consumed: theta, k of
 // slots 0 and 1 at xa + 1, their maps at xa, theta, k of slot 2 at xa, theta_max and dose
@@ -275,7 +277,7 @@ __global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
   __shared__ float sU[2][TY2 + 2][RW1];   // theta^n+1 of planes xa (b) and xa - 1 (1 - b)
   const int tid = threadIdx.x, tz = tid % TZ, ty = tid / TZ;
   const int y0 = blockIdx.y * TY2, z0 = blockIdx.x * TZ;
-  const int x0 = blockIdx.z * XCH2, x1 = min(x0 + XCH2, a.nx);
+  const int x0 = blockIdx.z * a.xch2, x1 = min(x0 + a.xch2, a.nx);
   const long plane = (long)a.ny * a.nz;
   const float iw = a.inv_dx2;
   // the thread's cells: slot 0 the tile cell, slot 1 a ring-1 cell, slot 2 a ring-2 cell
@@ -525,7 +527,12 @@ int cst_thermal_run(cst_thermal* t, int n_substeps) {
   dim3 grid((t->nz + TZ - 1) / TZ, (t->ny + TY - 1) / TY, (t->nx + XCH - 1) / XCH);
   int s = 0;
   // pairs of sub-steps: two steps per sweep over memory (k_pennes2)
-  const dim3 grid2((t->nz + TZ - 1) / TZ, (t->ny + TY2 - 1) / TY2, (t->nx + XCH2 - 1) / XCH2);
+  int ncu = 0;
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, t->device));
+  const long yz2 = (long)((t->nz + TZ - 1) / TZ) * ((t->ny + TY2 - 1) / TY2);
+  a.xch2 = 64;
+  while (a.xch2 > 8 && yz2 * ((t->nx + a.xch2 - 1) / a.xch2) < 2L * ncu) a.xch2 /= 2;
+  const dim3 grid2((t->nz + TZ - 1) / TZ, (t->ny + TY2 - 1) / TY2, (t->nx + a.xch2 - 1) / a.xch2);
   for (; s + 2 <= n_substeps; s += 2) {
     a.th_in = t->theta[t->cur];
     a.th_out = t->theta[1 - t->cur];
