@@ -364,7 +364,9 @@ __global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
       }
       kfm[c] = kfp;
     }
-    __syncthreads();
+    // no barrier: step n+2 reads only what the previous plane step wrote (sU[1 - b],
+    // sK[1 - b]) and its own cell's u_new, so the waves without a ring cell go on while
+    // the others finish theirs
     // step n+2 at plane xb for the tile cell (theta^n+1 of plane xb in sU[1 - b], of
     // planes xb -+ 1 in u_m and u_new)
     if (xb >= x0 && xb < x1 && in[0]) {
