@@ -120,7 +120,9 @@ def summarize_sq(a_dir: str, b_dir: str) -> str:
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
+    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2", "pennes"],
+                    help="k-space workload, or pennes: the thermal benchmark's 256^3 tissue "
+                         "heated by a seeded random source, --steps Pennes steps with the dose")
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--variant", type=int, default=0, help="z-pass thread layout (0 or 1)")
     ap.add_argument("--zu-variant", type=int, help="velocity-pass layout (0, 1 or 2; default --variant)")
@@ -147,6 +149,19 @@ def main() -> int:
     from caustica_amd.solvers.kspace.operators import plan_member
 
     get_backend("hip")
+    if args.workload == "pennes":
+        import numpy as np
+
+        from caustica_amd.thermal import PennesSolver
+
+        shape = (256, 256, 256)
+        q = (1e6 * np.random.default_rng(1).random(shape)).astype(np.float32)
+        print(f"[probe] pennes {shape}: {args.steps} steps", flush=True)
+        r = PennesSolver("hip", finite_check_every=args.steps).solve(
+            np.full(shape, 37.0, np.float32), q, workloads.pennes_tissue(shape, 3.75e-4), dt=0.05,
+            n_steps=args.steps, dose=True)
+        print(f"[probe] peak {float(r.temperature_max.max()):.3f} C", flush=True)
+        return 0
     wl = {"cfg0": workloads.cfg0_water_bowl, "cfg1": workloads.cfg1_hetero_linear,
           "cfg2": workloads.cfg2_westervelt}[args.workload]()
     plan = plan_member(wl.grid, wl.medium, wl.source, wl.spec, harmonics=wl.harmonics,
