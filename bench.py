@@ -332,6 +332,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3, help="timed solves")
     ap.add_argument("--warmup", type=int, default=1, help="untimed solves")
     ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
+    ap.add_argument("--batch", type=int, default=1,
+                    help="same-shape solves advanced together per launch (value = all solves/s; "
+                         "per-pass profile only at 1)")
     ap.add_argument("--cpu-steps", type=int, default=5, help="timed CPU baseline steps (0: skip)")
     ap.add_argument("--cpu-workers", type=int, default=1)
     ap.add_argument("--no-profile", action="store_true")
@@ -373,21 +376,27 @@ def main() -> int:
     rec_steps = wl.spec.n_record_periods * spp
     total_steps = wl.periods_total * spp
     record_from = total_steps - rec_steps
-    sess = KSpaceSession(plan.padded3, 2)
+    if not 1 <= args.batch <= 16:
+        raise SystemExit("--batch must be 1..16 (members one launch advances)")
+    sess = KSpaceSession(plan.padded3, max(2, args.batch))
     from caustica_amd.tune import autotune
 
     choice = autotune(sess, plan, use_cache=False)
     log(f"autotuned variants {choice}")
-    slot = sess.add(plan, max_periods=wl.periods_total + 4)
+    slots = [sess.add(plan, max_periods=wl.periods_total + 4) for _ in range(args.batch)]
+    slot = slots[0]
 
     def one_solve():
-        """Fresh solve of the resident member; the float64 phasor finalization runs on the
-        device, the result stays in HBM (no host copy inside the timed region)."""
-        sess.reset(slot)
-        sess.schedule(slot, record_from)
-        sess.advance([slot], total_steps)
-        for hi in range(len(plan.harmonics)):
-            N.check(N.lib().cst_member_result(sess.handle, slot, hi, 2.0 / rec_steps, None, None))
+        """Fresh solve of the resident member(s), all advanced by the same launches; the
+        float64 phasor finalization runs on the device, the result stays in HBM (no host
+        copy inside the timed region)."""
+        for s_ in slots:
+            sess.reset(s_)
+            sess.schedule(s_, record_from)
+        sess.advance(slots, total_steps)
+        for s_ in slots:
+            for hi in range(len(plan.harmonics)):
+                N.check(N.lib().cst_member_result(sess.handle, s_, hi, 2.0 / rec_steps, None, None))
 
     log(f"{wl.name}: spp {spp}, {total_steps} steps/solve; warmup {args.warmup}, timed {args.steps}")
     for _ in range(args.warmup):
@@ -402,7 +411,7 @@ def main() -> int:
         lat.append(time.perf_counter() - t0)
     device_sync(sess)
     elapsed = time.perf_counter() - t_start
-    value = args.steps / elapsed
+    value = args.batch * args.steps / elapsed
     peaks = sess.peaks(slot, 0, wl.periods_total - wl.spec.n_record_periods)
     if not np.isfinite(peaks).all():
         print("bench: non-finite field", file=sys.stderr)
@@ -410,7 +419,7 @@ def main() -> int:
 
     passes = {}
     roofline = None
-    if not args.no_profile:
+    if not args.no_profile and args.batch == 1:
         passes, step = profile_passes(sess, one_solve, plan, total_steps)
         dom = max(passes, key=lambda k: passes[k]["avg_us"] * passes[k]["launches"])
         d = passes[dom]
@@ -467,6 +476,7 @@ def main() -> int:
             "spp": spp,
             "steps_per_solve": total_steps,
             "harmonics": list(wl.harmonics),
+            "batch": args.batch,
         },
         "variants": {str(k): v for k, v in choice.items()},
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
