@@ -253,7 +253,6 @@ __device__ __forceinline__ void ring_cell(int r, int i, int& dy, int& dz) {
   }
 }
 
-
 // x planes per block (a.xch2): the longest of 64, 32, 16, 8 that still gives the grid at
 // least two blocks per CU (at 256^3: 64, 512 blocks in one round, the two recomputed
 // chunk-border planes 3 % of the work; smaller grids take shorter chunks rather than
@@ -280,13 +279,17 @@ __global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
   const int x0 = blockIdx.z * a.xch2, x1 = min(x0 + a.xch2, a.nx);
   const long plane = (long)a.ny * a.nz;
   const float iw = a.inv_dx2;
-  // the thread's cells: slot 0 the tile cell, slot 1 a ring-1 cell, slot 2 a ring-2 cell
+  // the thread's cells: slot 0 the tile cell, slot 1 a ring-1 cell (threads 0 .. NR1 - 1),
+  // slot 2 a ring-2 cell (the second half of the block: its loads and LDS writes go to
+  // other waves than the ring-1 updates)
+  constexpr int H2 = TZ * TY2 / 2;
+  static_assert(H2 >= NR1 && H2 >= NR2, "ring cells of the two halves do not share a thread");
   int cy[3], cz[3];
   cy[0] = ty;
   cz[0] = tz;
-  const bool has1 = tid < NR1, has2 = tid < NR2;
+  const bool has1 = tid < NR1, has2 = tid >= H2 && tid < H2 + NR2;
   ring_cell(1, has1 ? tid : 0, cy[1], cz[1]);
-  ring_cell(2, has2 ? tid : 0, cy[2], cz[2]);
+  ring_cell(2, has2 ? tid - H2 : 0, cy[2], cz[2]);
   bool in[3];
   long col[3];
 #pragma unroll
