@@ -354,6 +354,9 @@ def main() -> int:
             file=sys.stderr,
         )
         return 2
+    if not 1 <= args.batch <= 16:
+        print("bench.py: --batch must be 1..16 (the members one launch advances)", file=sys.stderr)
+        return 2
 
     import numpy as np
 
@@ -376,8 +379,6 @@ def main() -> int:
     rec_steps = wl.spec.n_record_periods * spp
     total_steps = wl.periods_total * spp
     record_from = total_steps - rec_steps
-    if not 1 <= args.batch <= 16:
-        raise SystemExit("--batch must be 1..16 (members one launch advances)")
     sess = KSpaceSession(plan.padded3, max(2, args.batch))
     from caustica_amd.tune import autotune
 
