@@ -13,7 +13,8 @@ Modules and what each restates (upstream file:line):
   sources.py  sources.py:105-175, analytic/geometry.py:15-64
   pennes.py   thermal/pennes.py:376-689, thermal/dose.py:74-89
   heating.py  sensors.py:193-295, sensors.py:356-435
-  analytic.py analytic/oneill.py:33-67, analytic/planewave.py:18-57
+  analytic.py analytic/oneill.py:33-67, analytic/planewave.py:18-57,
+              analytic/rayleigh.py:30-84
 
 Every function runs in one of two precisions:
   "fp32"  the upstream dtype and operation order (float32 state, complex64

@@ -1,10 +1,12 @@
 """Closed-form references used by the tests (test oracle; see oracle/__init__.py).
 
 Restates upstream ``src/caustica/analytic/oneill.py:33-80`` (O'Neil on-axis pressure of
-a focused bowl) and ``analytic/planewave.py:18-57`` (exponential attenuation, shock
-distance, Fubini harmonic amplitudes). These are textbook formulas; the tests pin them
-by their limits (focal gain k h, B1 -> 1 and B2 -> sigma/2 as sigma -> 0, Bessel series
-sum) before using them to pin the solvers.
+a focused bowl), ``analytic/planewave.py:18-57`` (exponential attenuation, shock
+distance, Fubini harmonic amplitudes) and ``analytic/rayleigh.py:30-84`` (Rayleigh-
+Sommerfeld integral of a sampled surface). These are textbook formulas; the tests pin
+them by their limits (focal gain k h, B1 -> 1 and B2 -> sigma/2 as sigma -> 0, Bessel
+series sum; the Rayleigh sum of a finely sampled cap reproduces O'Neil on the axis, and
+one point source gives rho c k v dS / (2 pi R)) before using them to pin the solvers.
 """
 
 from __future__ import annotations
@@ -36,3 +38,18 @@ def fubini(n: int, sigma):
     with np.errstate(invalid="ignore", divide="ignore"):
         b = 2.0 * jv(n, n * s) / (n * s)
     return np.where(s == 0, 1.0 if n == 1 else 0.0, b)
+
+
+def rayleigh_pressure(points, areas, v_n, targets, k: float, rho: float = 1000.0, c: float = 1500.0,
+                      chunk: int = 4096):
+    """Rayleigh-Sommerfeld CW pressure of a sampled source surface (e^{-i omega t} convention):
+    p(x) = -i rho c k / (2 pi) sum_j v_j dS_j exp(i k R_j) / R_j, R_j = |x - x_j|
+    (upstream analytic/rayleigh.py:30-84)."""
+    src = np.asarray(points, np.float64)
+    fld = np.atleast_2d(np.asarray(targets, np.float64))
+    strength = np.broadcast_to(np.asarray(v_n, np.complex128), (src.shape[0],)) * np.asarray(areas, np.float64)
+    out = np.empty(fld.shape[0], np.complex128)
+    for a in range(0, fld.shape[0], chunk):
+        r = np.sqrt(((fld[a:a + chunk, None, :] - src[None, :, :]) ** 2).sum(axis=-1))
+        out[a:a + chunk] = (np.exp(1j * k * r) / r) @ strength
+    return (-1j * rho * c * k / (2.0 * np.pi)) * out

@@ -8,7 +8,7 @@ phasor normalization, alpha/2 instead of alpha) fails one of them.
 import numpy as np
 import pytest
 
-from oracle.analytic import fubini, oneill_axial, shock_distance
+from oracle.analytic import fubini, oneill_axial, rayleigh_pressure, shock_distance
 from oracle.heating import heat_source
 from oracle.kspace import KSpaceOracle, RunSpec, derivative_factors, k_vectors, kappa_sinc
 from oracle.pennes import PennesOracle, cem43_rate
@@ -132,6 +132,25 @@ def test_oneill_limits():
     assert abs(oneill_axial(roc, a, roc, F0)[0]) == pytest.approx(1000 * C0 * k * h, rel=1e-12)
     z = roc * (1 - 1e-7)
     assert abs(oneill_axial(z, a, roc, F0)[0]) == pytest.approx(1000 * C0 * k * h, rel=1e-5)
+
+
+def test_rayleigh_point_source_and_cap_axis():
+    """One point source: |p| = rho c k v dS / (2 pi R) with phase k R - pi/2; a finely
+    sampled spherical cap reproduces O'Neil's on-axis closed form (its exact Rayleigh
+    integral) to sampling error."""
+    from oracle.sources import spherical_cap_points
+
+    k = 2 * np.pi * F0 / C0
+    r = 0.031
+    p = rayleigh_pressure(np.zeros((1, 3)), np.array([1e-6]), 0.5, np.array([[0.0, 0.0, r]]), k)[0]
+    assert abs(p) == pytest.approx(1000 * C0 * k * 0.5e-6 / (2 * np.pi * r), rel=1e-12)
+    assert np.angle(p * np.exp(-1j * k * r)) == pytest.approx(-np.pi / 2, abs=1e-12)
+    a, roc = 7.5e-3, 18e-3
+    pts, areas = spherical_cap_points(a, roc, C0 / F0 / 20)
+    z = np.linspace(0.3 * roc, 1.6 * roc, 40)
+    tg = np.column_stack([np.zeros_like(z), np.zeros_like(z), z])
+    ratio = np.abs(rayleigh_pressure(pts, areas, 1.0, tg, k)) / np.abs(oneill_axial(z, a, roc, F0))
+    assert np.abs(ratio - 1).max() < 1e-5
 
 
 def test_fubini_small_sigma_and_energy():
