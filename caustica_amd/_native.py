@@ -51,6 +51,8 @@ EXPORTS = (
     "cst_member_get_raw",
     "cst_member_set_raw",
     "cst_session_set_variant",
+    "cst_session_set_graphs",
+    "cst_session_graph_stats",
     "cst_session_set_profiling",
     "cst_session_kernel_time",
     "cst_thermal_create",
@@ -60,6 +62,7 @@ EXPORTS = (
     "cst_thermal_status",
     "cst_thermal_read",
     "cst_thermal_heat_from_member",
+    "cst_thermal_read_source",
     "cst_thermal_set_profiling",
     "cst_thermal_kernel_time",
 )
@@ -157,6 +160,8 @@ def _bind(lib):
         "cst_member_get_raw": ([v, i, p, C.c_size_t, C.POINTER(i64)], i),
         "cst_member_set_raw": ([v, i, p, C.c_size_t, i64], i),
         "cst_session_set_variant": ([v, i, i], i),
+        "cst_session_set_graphs": ([v, i], i),
+        "cst_session_graph_stats": ([v, C.POINTER(i), C.POINTER(i64), C.POINTER(i64)], i),
         "cst_session_set_profiling": ([v, i], i),
         "cst_session_kernel_time": ([v, i, C.POINTER(d), C.POINTER(i64)], i),
         "cst_thermal_create": ([i, i, i, i, C.POINTER(v)], i),
@@ -166,6 +171,7 @@ def _bind(lib):
         "cst_thermal_status": ([v, C.POINTER(i)], i),
         "cst_thermal_read": ([v, p, p, p], i),
         "cst_thermal_heat_from_member": ([v, v, i, i, p, p, p, d, C.c_float], i),
+        "cst_thermal_read_source": ([v, p], i),
         "cst_thermal_set_profiling": ([v, i], i),
         "cst_thermal_kernel_time": ([v, C.POINTER(d), C.POINTER(i64)], i),
     }

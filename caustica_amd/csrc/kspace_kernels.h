@@ -565,6 +565,12 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   TwStage<L, TC * T> tws;
   tws.load(twg);
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
+  // the first pass of a step advances the member's step clock (read by k_zp, which
+  // runs after this launch has completed); one thread of the member's first block
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const gptr<long long> ck = G(m.clock);
+    ck[0] = ck[0] + 1;
+  }
   KxStage<L, TC * T> kxs;
   kxs.load(m);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
@@ -1164,7 +1170,7 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   __shared__ int src_zl[kSrcCap];
   __shared__ float src_val[kSrcCap];
   const MemberDev& m = a.mem[a.ids[blockIdx.z]];
-  const StepMember& sm = a.sm[blockIdx.z];
+  const StepMember sm = update ? member_step(m) : StepMember{};
   const int nlines = a.s.nx * a.s.ny;
   const long line0 = (long)(gridDim.x - 1 - blockIdx.x) * Z::LPB;  // reverse order, as in k_zu
   const int nl = cmin(Z::LPB, nlines - (int)line0);
@@ -1203,6 +1209,8 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
     }
     z_stage_put<M, ZV>(stg, a.s, line0, nl, s);
     if (threadIdx.x <= nl) s.sptr[threadIdx.x] = sp_l;
+    // a block with as many lines as threads has one bound more than threads
+    for (int i = (int)threadIdx.x + Z::THREADS; i <= nl; i += Z::THREADS) s.sptr[i] = sptr[line0 + i];
 #pragma unroll
     for (int q = 0; q < kSrcPT; ++q) {
       const int j = sbeg + threadIdx.x + q * Z::THREADS;

@@ -22,7 +22,10 @@ constexpr int kPeakSlots = 64;
 constexpr int kPeakStride = 16;
 constexpr int kPeakWords = kPeakSlots * kPeakStride;  // per period
 
-struct MemberDev {
+// One member's descriptor, in a device table the kernels index by member slot. Entries
+// are whole 256-byte units, so that writing one member's entry (an admission while
+// other members' launches are in flight) never shares a cache line with another's.
+struct alignas(256) MemberDev {
   float* p;
   float2* sp;
   float2* sg[3];
@@ -50,6 +53,20 @@ struct MemberDev {
   float half_cdt;        // c_ref * dt / 2
   float inv_n;           // 1 / (nx ny nz): the inverse-transform normalization
   float inv_nz;          // 1 / nz: the z round trip of the velocity state
+  // Step clock, in device memory so that a time step's launches take the same
+  // arguments at every step (a captured step replays as it is): clock[0] counts the
+  // steps begun (the first pass of a step, k_xgrad, advances it), clock[1] is the
+  // first recorded step. The step scalars are derived from it on the device
+  // (member_step): drive amplitude from the ramp table, drive and harmonic phases from
+  // the spp-entry (cos, sin) table of 2 pi k / spp built in float64 on the host.
+  long long* clock;
+  const float* amp_tab;  // float32(amplitude * ramp envelope(n dt)) for n < ramp_n
+  const float2* trig;    // (cos, sin)(2 pi k / spp), k < spp
+  float amp_full;        // float32(amplitude) once the ramp is complete
+  int ramp_n;
+  int spp;
+  int max_periods;
+  int harm[kMaxHarmonics];
 };
 
 // Global-address-space view of a device pointer read from a table. Loads and stores
@@ -81,8 +98,29 @@ struct StepMember {
   int record;     // accumulate phasors / p_max this step
   float2 eh[kMaxHarmonics];  // exp(+i h 2 pi (n mod spp) / spp)
   int peak_idx;   // period slot for the convergence peak, -1: none
-  int pad_;
 };
+
+// Step scalars of the step in progress (engine.py:478-482 source drive, :535-550 record
+// window, :490-495 period peak), from the member's device clock. Called after the
+// step's k_xgrad has advanced clock[0].
+__device__ __forceinline__ StepMember member_step(const MemberDev& m) {
+  StepMember sm;
+  const gptr<const long long> ck = G((const long long*)m.clock);
+  const long long n = ck[0] - 1, record_from = ck[1];
+  const int spp = m.spp;
+  const int ph = (int)(n % spp);
+  const gptr<const float2> trig = G(m.trig);
+  const float2 cs = trig[ph];
+  sm.cos_t = cs.x;
+  sm.sin_t = cs.y;
+  sm.amp = n < m.ramp_n ? G(m.amp_tab)[n] : m.amp_full;
+  sm.record = n >= record_from ? 1 : 0;
+#pragma unroll
+  for (int h = 0; h < kMaxHarmonics; ++h) sm.eh[h] = trig[(int)(((long long)m.harm[h] * ph) % spp)];
+  const long long pidx = n / spp;
+  sm.peak_idx = (!sm.record && pidx < m.max_periods) ? (int)pidx : -1;
+  return sm;
+}
 
 // Half-spectrum layout (nzh = nz/2 + 1 bins per (x, y)): bins k < nz/2 of (x, y) sit in
 // rows at x ps + y ms + k; the Nyquist bin k = nz/2 of every (x, y) sits in a side plane
@@ -102,7 +140,6 @@ struct StepArgs {
   Shape s;
   int nmem;
   int ids[kMaxBatch];
-  StepMember sm[kMaxBatch];
 };
 
 }  // namespace cst

@@ -610,6 +610,17 @@ int cst_thermal_read(cst_thermal* t, float* theta, float* theta_max, float* dose
   return CST_OK;
 }
 
+int cst_thermal_read_source(cst_thermal* t, float* q) {
+  if (!t || !q) return fail(CST_ERR_ARG, "null argument");
+  if (!t->ready) return fail(CST_ERR_STATE, "thermal session not set up");
+  if (!t->have_q) return fail(CST_ERR_STATE, "thermal session has no heat source");
+  const long P = (long)t->nx * t->ny * t->nz;
+  HIPCHK(hipSetDevice(t->device));
+  HIPCHK(hipMemcpyAsync(q, t->q, P * 4, hipMemcpyDeviceToHost, t->stream));
+  HIPCHK(hipStreamSynchronize(t->stream));
+  return CST_OK;
+}
+
 int cst_thermal_set_profiling(cst_thermal* t, int on) {
   if (!t) return fail(CST_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(t->device));

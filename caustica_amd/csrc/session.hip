@@ -1,6 +1,13 @@
-// Host session of the CW k-space solver: device memory per member, step scalars,
-// batched launch sequence, results, checkpoint state, kernel timing. C ABI in
-// include/caustica_hip.h.
+// Host session of the CW k-space solver: device memory per member, step clocks,
+// batched launch sequence (direct or replayed from captured HIP graphs), results,
+// checkpoint state, kernel timing. C ABI in include/caustica_hip.h.
+//
+// Streams: every step of a session runs on its compute stream, in order; member
+// uploads run on an upload stream (the host waits for them before cst_member_add
+// returns, so nothing on the compute stream has to wait); read-backs (period peaks,
+// finalized phasors) run on an io stream that first waits for the member's last
+// queued step (an event recorded after each cst_advance), so that reading a finished
+// member back overlaps the next steps of the others.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,11 +36,22 @@ struct Member {
   int max_periods = 0;
   int64_t rec_count = 0;
   bool finalized[kMaxHarmonics] = {false, false, false};
+  double final_scale[kMaxHarmonics] = {0, 0, 0};
   bool primed = true;
   void* block = nullptr;
   size_t block_bytes = 0;
+  hipEvent_t done = nullptr;  // recorded after the member's last queued step
   MemberDev dev{};
 };
+
+// Sets a member's device step clock (which bit 0: steps begun, bit 1: first recorded
+// step). The values travel as kernel arguments: no host buffer has to outlive the call.
+__global__ void k_clock_set(long long* clock, long long n, long long record_from, int which) {
+  if (threadIdx.x == 0) {
+    if (which & 1) clock[0] = n;
+    if (which & 2) clock[1] = record_from;
+  }
+}
 
 __global__ void k_finalize(const float2* __restrict__ acc, float2* __restrict__ comp, int64_t n,
                            double scale) {
@@ -72,11 +90,28 @@ __global__ void __launch_bounds__(256) k_heat_join(HeatJoin j) {
 
 }  // namespace
 
+// A captured launch sequence of `steps` time steps for one member list.
+struct GraphEntry {
+  int nmem = 0;
+  int ids[cst::kMaxBatch] = {};
+  int v2 = 0, v3 = 0;
+  bool nonlinear = false;
+  int steps = 0;
+  hipGraphExec_t exec = nullptr;
+  uint64_t last_use = 0;
+};
+
 struct cst_session {
   int device = 0;
   Shape s{};
   int max_members = 0;
   hipStream_t stream = nullptr;
+  hipStream_t up = nullptr;  // member uploads
+  hipStream_t io = nullptr;  // read-backs
+  bool graphs = true;
+  std::vector<GraphEntry> gcache;
+  uint64_t gclock = 0;
+  int64_t graph_launches = 0, graph_captures = 0;
   std::vector<Member> members;
   MemberDev* d_table = nullptr;
   float2* d_tables = nullptr;  // twiddles: x (nx), y (ny), z (M), real split (M + 1)
@@ -106,25 +141,42 @@ int check_slot(cst_session* s, int slot) {
   return CST_OK;
 }
 
-void step_scalars(const Member& m, StepMember& sm) {
-  const int64_t n = m.n;
-  const double period = 1.0 / m.f0;
-  const double t = (double)n * m.dt;
-  const double x = std::min(t / (m.ramp_periods * period), 1.0);
-  const double env = 0.5 * (1.0 - std::cos(M_PI * x));
-  sm.amp = (float)(m.amplitude * env);
-  const int64_t ph = n % m.spp;
-  const double th = 2.0 * M_PI * (double)ph / (double)m.spp;
-  sm.sin_t = (float)std::sin(th);
-  sm.cos_t = (float)std::cos(th);
-  sm.record = n >= m.record_from ? 1 : 0;
-  for (int h = 0; h < kMaxHarmonics; ++h) {
-    const int hh = h < m.nh ? m.harmonics[h] : 0;
-    const double a = 2.0 * M_PI * (double)((hh * ph) % m.spp) / (double)m.spp;
-    sm.eh[h] = make_float2((float)std::cos(a), (float)std::sin(a));
+// Drive amplitude float32(amplitude * env(n dt)) with env = (1 - cos(pi min(t / (ramp T),
+// 1))) / 2 (sources.py:98-102), in float64, for the steps before the ramp completes; the
+// device takes float32(amplitude) from then on (env = 1 exactly).
+std::vector<float> ramp_table(double amplitude, double dt, double f0, double ramp_periods) {
+  std::vector<float> out;
+  const double period = 1.0 / f0;
+  for (int64_t n = 0;; ++n) {
+    const double x = std::min((double)n * dt / (ramp_periods * period), 1.0);
+    if (x >= 1.0) break;
+    out.push_back((float)(amplitude * (0.5 * (1.0 - std::cos(M_PI * x)))));
   }
-  const int64_t pidx = n / m.spp;
-  sm.peak_idx = (!sm.record && pidx < m.max_periods) ? (int)pidx : -1;
+  return out;
+}
+
+// (cos, sin)(2 pi k / spp) for k < spp in float64, rounded to float32: the drive phase
+// 2 pi (n mod spp) / spp and every harmonic's record phase 2 pi (h n mod spp) / spp.
+std::vector<float2> trig_table(int spp) {
+  std::vector<float2> out(spp);
+  for (int k = 0; k < spp; ++k) {
+    const double th = 2.0 * M_PI * (double)k / (double)spp;
+    out[k] = make_float2((float)std::cos(th), (float)std::sin(th));
+  }
+  return out;
+}
+
+int set_clock(cst_session* s, Member& m, int64_t n, int64_t record_from, int which) {
+  hipLaunchKernelGGL(k_clock_set, dim3(1), dim3(64), 0, s->stream, m.dev.clock, (long long)n,
+                     (long long)record_from, which);
+  HIPCHK(hipGetLastError());
+  return CST_OK;
+}
+
+bool batch_nonlinear(const cst_session* s, const StepArgs& a) {
+  for (int i = 0; i < a.nmem; ++i)
+    if (s->members[a.ids[i]].dev.beta2dt != nullptr) return true;
+  return false;
 }
 
 int launch_step(cst_session* s, StepArgs& a) {
@@ -139,8 +191,7 @@ int launch_step(cst_session* s, StepArgs& a) {
   ok = ok && s->prof.run(1, st, [&] { return launch_ydiv(sh.ny, a, s->tw_y, st); });
   ok = ok && s->prof.run(3, st, [&] { return launch_xdiv(sh.nx, a, s->tw_x, st); });
   if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 0, st); });
-  bool nonlinear = false;
-  for (int i = 0; i < a.nmem; ++i) nonlinear = nonlinear || s->members[a.ids[i]].dev.beta2dt != nullptr;
+  const bool nonlinear = batch_nonlinear(s, a);
   ok = ok && s->prof.run(5, st, [&] {
     return launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 1, nonlinear, st);
   });
@@ -150,14 +201,87 @@ int launch_step(cst_session* s, StepArgs& a) {
   return CST_OK;
 }
 
+// A new state restarts the record window: the sums and p_max are cleared (checkpoints
+// are taken before the first record step, engine.py:344-371) and the clock set to n.
+int restart_record(cst_session* s, Member& m, int64_t n) {
+  HIPCHK(hipStreamSynchronize(s->io));
+  for (int h = 0; h < m.nh; ++h) {
+    HIPCHK(hipMemsetAsync(m.dev.acc[h], 0, m.rec_count * 8, s->stream));
+    HIPCHK(hipMemsetAsync(m.dev.comp[h], 0, m.rec_count * 8, s->stream));
+    m.finalized[h] = false;
+  }
+  HIPCHK(hipMemsetAsync(m.dev.pmax, 0, m.rec_count * 4, s->stream));
+  return set_clock(s, m, n, 0, 1);
+}
+
 StepArgs single_member(const cst_session* s, int slot) {
   StepArgs a{};
   a.mem = s->d_table;
   a.s = s->s;
   a.nmem = 1;
   a.ids[0] = slot;
-  a.sm[0].peak_idx = -1;
   return a;
+}
+
+void destroy_graphs(cst_session* s) {
+  if (s->gcache.empty()) return;
+  (void)hipStreamSynchronize(s->stream);
+  for (auto& g : s->gcache) (void)hipGraphExecDestroy(g.exec);
+  s->gcache.clear();
+}
+
+constexpr int kGraphSteps = 16;   // steps per replayed graph (a remainder replays 1-step graphs)
+constexpr size_t kGraphCache = 48;
+
+// The cached executable graph of `steps` steps for this member list and variant set,
+// captured on first use; nullptr (and graphs off for the session) if capture fails.
+hipGraphExec_t graph_for(cst_session* s, StepArgs& a, int steps) {
+  const bool nl = batch_nonlinear(s, a);
+  for (auto& g : s->gcache) {
+    if (g.nmem != a.nmem || g.steps != steps || g.v2 != s->variant[2] || g.v3 != s->variant[3] ||
+        g.nonlinear != nl)
+      continue;
+    if (!std::equal(a.ids, a.ids + a.nmem, g.ids)) continue;
+    g.last_use = ++s->gclock;
+    return g.exec;
+  }
+  if (s->gcache.size() >= kGraphCache) {
+    auto it = std::min_element(s->gcache.begin(), s->gcache.end(),
+                               [](const GraphEntry& x, const GraphEntry& y) { return x.last_use < y.last_use; });
+    (void)hipStreamSynchronize(s->stream);  // the evicted graph may still be running
+    (void)hipGraphExecDestroy(it->exec);
+    s->gcache.erase(it);
+  }
+  hipGraph_t graph = nullptr;
+  if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    s->graphs = false;
+    return nullptr;
+  }
+  int r = CST_OK;
+  for (int k = 0; k < steps && r == CST_OK; ++k) r = launch_step(s, a);
+  const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  hipGraphExec_t exec = nullptr;
+  bool ok = r == CST_OK && e == hipSuccess && graph != nullptr &&
+            hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!ok) {
+    (void)hipGetLastError();
+    s->graphs = false;
+    return nullptr;
+  }
+  GraphEntry g;
+  g.nmem = a.nmem;
+  std::copy(a.ids, a.ids + a.nmem, g.ids);
+  g.v2 = s->variant[2];
+  g.v3 = s->variant[3];
+  g.nonlinear = nl;
+  g.steps = steps;
+  g.exec = exec;
+  g.last_use = ++s->gclock;
+  s->gcache.push_back(g);
+  s->graph_captures++;
+  return exec;
 }
 
 // R2C + forward y of the member's current p into sp (after set_state).
@@ -186,7 +310,7 @@ __global__ void __launch_bounds__(256) k_diag_copy(const float4* __restrict__ sr
 
 extern "C" {
 
-int cst_version(void) { return 101; }
+int cst_version(void) { return 102; }
 
 int cst_last_error(char* buf, size_t n) { return copy_last_error(buf, n); }
 
@@ -222,8 +346,10 @@ int cst_session_create(int nx, int ny, int nz, int max_members, int device, cst_
   s->max_members = max_members;
   s->members.resize(max_members);
   hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->io, hipStreamNonBlocking);
   if (e != hipSuccess) {
-    delete s;
+    cst_session_destroy(s);
     return fail(CST_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
   }
   std::vector<float2> tab;
@@ -255,13 +381,20 @@ int cst_session_destroy(cst_session* s) {
   if (!s) return CST_OK;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  for (auto& m : s->members)
+  if (s->io) (void)hipStreamSynchronize(s->io);
+  if (s->up) (void)hipStreamSynchronize(s->up);
+  destroy_graphs(s);
+  for (auto& m : s->members) {
     if (m.block) (void)hipFree(m.block);
+    if (m.done) (void)hipEventDestroy(m.done);
+  }
   for (auto& b : s->pool) (void)hipFree(b.second);
   if (s->d_table) (void)hipFree(s->d_table);
   if (s->d_tables) (void)hipFree(s->d_tables);
   s->prof.release();
   if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->up) (void)hipStreamDestroy(s->up);
+  if (s->io) (void)hipStreamDestroy(s->io);
   delete s;
   return CST_OK;
 }
@@ -334,6 +467,9 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   const int64_t nk = 2LL * (sh.nx + sh.ny + sh.nzh);
   const int64_t nsp = sh.nz + sh.nx + sh.ny;
   const int maxp = std::max(d->max_periods, 1);
+  const std::vector<float> amp_tab = ramp_table(d->amplitude, d->dt, d->f0, d->ramp_periods);
+  const std::vector<float2> trig = trig_table(d->spp);
+  const long long clock0[2] = {0, INT64_MAX};
   Carver c;
   const size_t o_state = c.take(P * 4);      // p
   const size_t o_spec = c.take(7 * H * 8);   // sp, sg0..sg2, su0..su2 (velocity state)
@@ -347,6 +483,9 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   const size_t o_rec = c.take(2 * d->n_harmonics * R * 8);
   const size_t o_pmax = c.take(R * 4);
   const size_t o_peaks = c.take((int64_t)maxp * kPeakWords * 4);
+  const size_t o_clock = c.take(sizeof(clock0));
+  const size_t o_amp = c.take((int64_t)std::max<size_t>(amp_tab.size(), 1) * 4);
+  const size_t o_trig = c.take((int64_t)trig.size() * 8);
   HIPCHK(hipSetDevice(s->device));
   Member& m = s->members[slot];
   char* base = nullptr;
@@ -357,7 +496,16 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
       break;
     }
   if (!base) HIPCHK(hipMalloc((void**)&base, c.total));
+  hipEvent_t done = m.done;  // the slot's event outlives its members
+  if (!done) {
+    const hipError_t ee = hipEventCreateWithFlags(&done, hipEventDisableTiming);
+    if (ee != hipSuccess) {
+      s->pool.emplace_back(c.total, base);
+      return fail(CST_ERR_HIP, std::string("hipEventCreate: ") + hipGetErrorString(ee));
+    }
+  }
   m = Member{};
+  m.done = done;
   m.block = base;
   m.block_bytes = c.total;
   m.used = true;
@@ -389,6 +537,14 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
     md.rec[i] = d->rec[i];
   }
   md.nh = d->n_harmonics;
+  md.clock = (long long*)(base + o_clock);
+  md.amp_tab = (const float*)(base + o_amp);
+  md.trig = (const float2*)(base + o_trig);
+  md.amp_full = (float)(d->amplitude * (0.5 * (1.0 - std::cos(M_PI * 1.0))));
+  md.ramp_n = (int)amp_tab.size();
+  md.spp = d->spp;
+  md.max_periods = maxp;
+  for (int h = 0; h < kMaxHarmonics; ++h) md.harm[h] = h < d->n_harmonics ? d->harmonics[h] : 0;
   md.half_cdt = (float)(0.5 * d->c_ref * d->dt);
   md.inv_n = (float)(1.0 / (double)P);
   md.inv_nz = (float)(1.0 / (double)sh.nz);
@@ -402,7 +558,9 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   for (int h = 0; h < kMaxHarmonics; ++h) m.harmonics[h] = h < d->n_harmonics ? d->harmonics[h] : 0;
   m.max_periods = maxp;
 
-  hipStream_t stm = s->stream;
+  // uploads on the upload stream: members of the session may be running on the
+  // compute stream meanwhile (the block and the table entry are this member's alone)
+  hipStream_t stm = s->up;
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stm);
   };
@@ -420,11 +578,15 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   if (e == hipSuccess) e = hipMemsetAsync(base + o_rec, 0, 2 * d->n_harmonics * R * 8, stm);
   if (e == hipSuccess) e = hipMemsetAsync(base + o_pmax, 0, R * 4, stm);
   if (e == hipSuccess) e = hipMemsetAsync(md.peaks, 0, (size_t)maxp * kPeakWords * 4, stm);
+  if (e == hipSuccess) e = up(md.clock, clock0, sizeof(clock0));
+  if (e == hipSuccess && !amp_tab.empty()) e = up((void*)md.amp_tab, amp_tab.data(), amp_tab.size() * 4);
+  if (e == hipSuccess) e = up((void*)md.trig, trig.data(), trig.size() * 8);
   if (e == hipSuccess) e = up(s->d_table + slot, &md, sizeof(MemberDev));
   if (e == hipSuccess) e = hipStreamSynchronize(stm);
   if (e != hipSuccess) {
     s->pool.emplace_back(c.total, base);
     m = Member{};
+    m.done = done;
     return fail(CST_ERR_HIP, std::string("member upload: ") + hipGetErrorString(e));
   }
   m.rec_count = R;
@@ -436,9 +598,12 @@ int cst_member_remove(cst_session* s, int slot) {
   if (int r = check_slot(s, slot)) return r;
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipStreamSynchronize(s->io));
   // keep the block for the next member of this shape (hipFree would stall every stream)
   s->pool.emplace_back(s->members[slot].block_bytes, s->members[slot].block);
+  hipEvent_t done = s->members[slot].done;
   s->members[slot] = Member{};
+  s->members[slot].done = done;
   return CST_OK;
 }
 
@@ -509,6 +674,7 @@ int cst_member_reset(cst_session* s, int slot) {
   }
   HIPCHK(hipMemsetAsync(m.dev.pmax, 0, m.rec_count * 4, s->stream));
   HIPCHK(hipMemsetAsync(m.dev.peaks, 0, (size_t)m.max_periods * kPeakWords * 4, s->stream));
+  if (int r = set_clock(s, m, 0, INT64_MAX, 3)) return r;
   m.n = 0;
   m.record_from = INT64_MAX;
   m.primed = true;
@@ -517,6 +683,8 @@ int cst_member_reset(cst_session* s, int slot) {
 
 int cst_member_schedule(cst_session* s, int slot, int64_t record_from) {
   if (int r = check_slot(s, slot)) return r;
+  HIPCHK(hipSetDevice(s->device));
+  if (int r = set_clock(s, s->members[slot], 0, record_from, 2)) return r;
   s->members[slot].record_from = record_from;
   return CST_OK;
 }
@@ -537,6 +705,14 @@ int cst_advance(cst_session* s, int n_slots, const int* slots, int n_steps) {
     for (int j = 0; j < i; ++j)
       if (slots[j] == slots[i]) return fail(CST_ERR_ARG, "member listed twice");
   }
+  for (int i = 0; i < n_slots; ++i) {
+    const Member& m = s->members[slots[i]];
+    for (int h = 0; h < m.nh; ++h)
+      if (m.finalized[h])
+        return fail(CST_ERR_STATE, "member " + std::to_string(slots[i]) +
+                                       ": its phasors were finalized by cst_member_result; reset it "
+                                       "(cst_member_reset) or give it a new state before advancing it");
+  }
   HIPCHK(hipSetDevice(s->device));
   for (int i = 0; i < n_slots; ++i)
     if (!s->members[slots[i]].primed)
@@ -546,15 +722,33 @@ int cst_advance(cst_session* s, int n_slots, const int* slots, int n_steps) {
   a.s = s->s;
   a.nmem = n_slots;
   for (int i = 0; i < n_slots; ++i) a.ids[i] = slots[i];
-  for (int k = 0; k < n_steps; ++k) {
-    for (int i = 0; i < n_slots; ++i) {
-      Member& m = s->members[slots[i]];
-      step_scalars(m, a.sm[i]);
-      if (a.sm[i].record)
-        for (int h = 0; h < m.nh; ++h) m.finalized[h] = false;
+  // every step of the sequence takes the same arguments (the step scalars come from the
+  // members' device clocks): replay captured graphs unless the launches are being timed
+  int k = 0;
+  if (s->graphs && !s->prof.on && n_steps >= 2) {
+    if (n_steps >= kGraphSteps) {
+      if (hipGraphExec_t g = graph_for(s, a, kGraphSteps)) {
+        for (; k + kGraphSteps <= n_steps; k += kGraphSteps) {
+          HIPCHK(hipGraphLaunch(g, s->stream));
+          s->graph_launches++;
+        }
+      }
     }
+    if (s->graphs && k < n_steps) {
+      if (hipGraphExec_t g = graph_for(s, a, 1)) {
+        for (; k < n_steps; ++k) {
+          HIPCHK(hipGraphLaunch(g, s->stream));
+          s->graph_launches++;
+        }
+      }
+    }
+  }
+  for (; k < n_steps; ++k)
     if (int r = launch_step(s, a)) return r;
-    for (int i = 0; i < n_slots; ++i) s->members[slots[i]].n++;
+  for (int i = 0; i < n_slots; ++i) {
+    Member& m = s->members[slots[i]];
+    m.n += n_steps;
+    HIPCHK(hipEventRecord(m.done, s->stream));
   }
   return CST_OK;
 }
@@ -588,9 +782,10 @@ int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out)
   if (count == 0) return CST_OK;
   HIPCHK(hipSetDevice(s->device));
   std::vector<unsigned> raw((size_t)count * kPeakWords);
+  HIPCHK(hipStreamWaitEvent(s->io, m.done, 0));
   HIPCHK(hipMemcpyAsync(raw.data(), m.dev.peaks + (size_t)first * kPeakWords,
-                        raw.size() * 4, hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
+                        raw.size() * 4, hipMemcpyDeviceToHost, s->io));
+  HIPCHK(hipStreamSynchronize(s->io));
   for (int i = 0; i < count; ++i) {
     unsigned b = 0u;
     for (int k = 0; k < kPeakSlots; ++k) b = std::max(b, raw[(size_t)i * kPeakWords + k * kPeakStride]);
@@ -603,23 +798,27 @@ int cst_member_result(cst_session* s, int slot, int hi, double scale, float* pha
   if (int r = check_slot(s, slot)) return r;
   Member& m = s->members[slot];
   if (hi < 0 || hi >= m.nh) return fail(CST_ERR_ARG, "harmonic index out of range");
+  if (m.finalized[hi] && m.final_scale[hi] != scale)
+    return fail(CST_ERR_ARG, "harmonic " + std::to_string(hi) +
+                                 " was already finalized with another scale (the reduction runs once)");
   HIPCHK(hipSetDevice(s->device));
   const int64_t R = m.rec_count;
+  HIPCHK(hipStreamWaitEvent(s->io, m.done, 0));
   if (!m.finalized[hi]) {
     const int threads = 256;
     const int64_t blocks = (R + threads - 1) / threads;
-    s->prof.run(6, s->stream, [&] {
-      hipLaunchKernelGGL(k_finalize, dim3((unsigned)blocks), dim3(threads), 0, s->stream, m.dev.acc[hi],
+    s->prof.run(6, s->io, [&] {
+      hipLaunchKernelGGL(k_finalize, dim3((unsigned)blocks), dim3(threads), 0, s->io, m.dev.acc[hi],
                          m.dev.comp[hi], R, scale);
       return true;
     });
     HIPCHK(hipGetLastError());
     m.finalized[hi] = true;
+    m.final_scale[hi] = scale;
   }
-  if (phasor)
-    HIPCHK(hipMemcpyAsync(phasor, m.dev.comp[hi], R * 8, hipMemcpyDeviceToHost, s->stream));
-  if (pmax) HIPCHK(hipMemcpyAsync(pmax, m.dev.pmax, R * 4, hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
+  if (phasor) HIPCHK(hipMemcpyAsync(phasor, m.dev.comp[hi], R * 8, hipMemcpyDeviceToHost, s->io));
+  if (pmax) HIPCHK(hipMemcpyAsync(pmax, m.dev.pmax, R * 4, hipMemcpyDeviceToHost, s->io));
+  HIPCHK(hipStreamSynchronize(s->io));
   return CST_OK;
 }
 
@@ -661,6 +860,7 @@ int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, co
       return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
     HIPCHK(hipGetLastError());
   }
+  if (int r = restart_record(s, m, n)) return r;
   HIPCHK(hipStreamSynchronize(s->stream));
   m.n = n;
   m.primed = false;
@@ -704,6 +904,7 @@ int cst_member_set_raw(cst_session* s, int slot, const void* buf, size_t bytes, 
   HIPCHK(hipMemcpyAsync(m.dev.p, src, pb, hipMemcpyHostToDevice, s->stream));
   for (int ax = 0; ax < 3; ++ax)
     HIPCHK(hipMemcpyAsync(m.dev.su[ax], src + pb + ax * hb, hb, hipMemcpyHostToDevice, s->stream));
+  if (int r = restart_record(s, m, n)) return r;
   HIPCHK(hipStreamSynchronize(s->stream));
   m.n = n;
   m.primed = false;
@@ -715,6 +916,22 @@ int cst_session_set_variant(cst_session* s, int pass, int variant) {
   if (pass < 0 || pass > 3 || variant < 0 || variant > (pass == 2 ? 2 : 1))
     return fail(CST_ERR_ARG, "pass must be 0..3 and variant 0..1 (0..2 for the velocity pass)");
   s->variant[pass] = variant;
+  return CST_OK;
+}
+
+int cst_session_set_graphs(cst_session* s, int on) {
+  if (!s) return fail(CST_ERR_ARG, "null session");
+  HIPCHK(hipSetDevice(s->device));
+  destroy_graphs(s);
+  s->graphs = on != 0;
+  return CST_OK;
+}
+
+int cst_session_graph_stats(cst_session* s, int* enabled, int64_t* launches, int64_t* captures) {
+  if (!s || !enabled || !launches || !captures) return fail(CST_ERR_ARG, "null argument");
+  *enabled = s->graphs ? 1 : 0;
+  *launches = s->graph_launches;
+  *captures = s->graph_captures;
   return CST_OK;
 }
 
@@ -761,7 +978,8 @@ int cst_thermal_heat_from_member(cst_thermal* t, cst_session* s, int slot, int n
   j.scale = scale;
   j.q = t->q;
   HIPCHK(hipSetDevice(s->device));
-  HIPCHK(hipStreamSynchronize(s->stream));  // the finalized phasors are complete
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipStreamSynchronize(s->io));  // the finalized phasors are complete
   // w is staged in the thermal theta[1] buffer (free until the first thermal sub-step)
   float* wd = t->theta[1 - t->cur];
   HIPCHK(hipMemcpyAsync(wd, w, (size_t)j.n * 4, hipMemcpyHostToDevice, t->stream));

@@ -132,8 +132,18 @@ def _al(n: int) -> int:
     return (max(int(n), 0) + 255) // 256 * 256
 
 
+def ramp_steps(dt: float, f0: float, ramp_periods: float) -> int:
+    """Steps before the cosine ramp completes (the length of the library's drive-amplitude
+    table): the n with n dt / (ramp_periods / f0) < 1, evaluated as the library does."""
+    period = 1.0 / f0
+    n = 0
+    while min(n * dt / (ramp_periods * period), 1.0) < 1.0:
+        n += 1
+    return n
+
+
 def hip_inventory(padded3, *, nonlinear: bool, n_harmonics: int, rec_elems: int,
-                  n_src: int, max_periods: int) -> MemoryModel:
+                  n_src: int, max_periods: int, spp: int = 1, ramp_n: int = 0) -> MemoryModel:
     """Bytes of one member on the hip engine, piece by piece as the library carves them."""
     nx, ny, nz = (int(v) for v in padded3)
     nzh = nz // 2 + 1
@@ -150,6 +160,7 @@ def hip_inventory(padded3, *, nonlinear: bool, n_harmonics: int, rec_elems: int,
         "source table": _al((nx * ny + 1) * 4) + _al(ns * 4) + _al(ns * 8),
         "record buffers": _al(2 * n_harmonics * rec_elems * 8) + _al(rec_elems * 4),
         "convergence peaks": _al(max(int(max_periods), 1) * 64 * 16 * 4),
+        "step clock and drive tables": _al(16) + _al(max(int(ramp_n), 1) * 4) + _al(int(spp) * 8),
     }
     return MemoryModel(b, int(sum(b.values())), tuple(padded3))
 
@@ -160,7 +171,8 @@ def hip_memory(plan: MemberPlan, max_periods: int | None = None) -> MemoryModel:
     return hip_inventory(plan.padded3, nonlinear=plan.beta2_dt is not None,
                          n_harmonics=len(plan.harmonics), rec_elems=int(np.prod(plan.rec_shape)),
                          n_src=int(plan.src_idx3.shape[0]),
-                         max_periods=plan.eff_max + 4 if max_periods is None else max_periods)
+                         max_periods=plan.eff_max + 4 if max_periods is None else max_periods,
+                         spp=plan.spp, ramp_n=ramp_steps(plan.dt, plan.f0, plan.ramp_periods))
 
 
 # ------------------------------------------------------------------- time
@@ -276,7 +288,8 @@ def estimate(grid, medium, source, spec=None, *, solver: str = "westervelt",
     if dev.runs_hip:
         mem = hip_inventory(padded3, nonlinear=nonlinear, n_harmonics=len(harmonics),
                             rec_elems=rec_elems, n_src=int(source.indices.shape[0]),
-                            max_periods=pre_worst + spec.n_record_periods + 4)
+                            max_periods=pre_worst + spec.n_record_periods + 4, spp=spp,
+                            ramp_n=ramp_steps(dt, source.f0, source.ramp_periods))
         entry = find_calibration_for(dev.key, padded3, nonlinear, calibration_path)
         if entry is not None:
             rate, warm, label = entry["bytes_per_s"], entry["warmup_s"], "calibrated"

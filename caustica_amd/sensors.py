@@ -253,6 +253,39 @@ class DeviceHeatingSource:
     def shape(self):
         return tuple(self.w.shape)
 
+    def to_host(self) -> np.ndarray:
+        """Q [W/m^3] as the device forms it (float32 over the region), read back: the join
+        runs into a scratch thermal session of the region's shape (no perfusion offset)
+        and cst_thermal_read_source copies it out. For inspection and tests; a Pennes
+        solve never needs it."""
+        import ctypes as C
+
+        from caustica_amd import _native as N
+
+        lib = N.lib()
+        shape = self.shape
+        shape3 = (1,) * (3 - len(shape)) + tuple(shape)
+        zero = np.zeros(shape, np.float32)
+        desc = N.ThermalDesc()
+        desc.dx = self.dx
+        desc.k = desc.perf = desc.coeff = desc.theta0 = N.ptr(zero)
+        desc.q = desc.dose0 = None
+        desc.half_dt_min = 0.0
+        h = C.c_void_p()
+        N.check(lib.cst_thermal_create(*shape3, self.session.device, C.byref(h)))
+        try:
+            N.check(lib.cst_thermal_setup(h, C.byref(desc)))
+            idx = np.array([t[0] for t in self.terms], np.int32)
+            fac = np.array([t[1] for t in self.terms], np.float64)
+            N.check(lib.cst_thermal_heat_from_member(h, self.session.handle, self.slot, len(idx),
+                                                     N.ptr(idx), N.ptr(fac), N.ptr(self.w),
+                                                     float(self.scale), 0.0))
+            out = np.empty(shape, np.float32)
+            N.check(lib.cst_thermal_read_source(h, N.ptr(out)))
+        finally:
+            lib.cst_thermal_destroy(h)
+        return out
+
     def scaled(self, factor: float) -> DeviceHeatingSource:
         """Same distribution, Q multiplied by ``factor`` (applied in float64 on the device)."""
         return dataclasses.replace(self, scale=self.scale * float(factor),

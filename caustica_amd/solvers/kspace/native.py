@@ -20,6 +20,7 @@ KERNEL_KINDS = ("xgrad", "y3", "zu", "xdiv", "y1", "zp", "finalize")
 class KSpaceSession:
     def __init__(self, padded3, max_members: int = 1, device: int = 0):
         self.padded3 = tuple(int(v) for v in padded3)
+        self.device = int(device)
         self._lib = N.lib()
         h = C.c_void_p()
         N.check(self._lib.cst_session_create(*self.padded3, int(max_members), int(device), C.byref(h)))
@@ -197,6 +198,15 @@ class KSpaceSession:
 
     def set_variant(self, pass_: int, variant: int) -> None:
         N.check(self._lib.cst_session_set_variant(self._h, int(pass_), int(variant)))
+
+    def set_graphs(self, on: bool) -> None:
+        """Replay captured HIP graphs of the step sequence (default on) or launch each pass."""
+        N.check(self._lib.cst_session_set_graphs(self._h, 1 if on else 0))
+
+    def graph_stats(self) -> dict:
+        en, la, ca = C.c_int(), C.c_int64(), C.c_int64()
+        N.check(self._lib.cst_session_graph_stats(self._h, C.byref(en), C.byref(la), C.byref(ca)))
+        return {"enabled": bool(en.value), "launches": int(la.value), "captures": int(ca.value)}
 
     # ---------------------------------------------------------------- timing
     def set_profiling(self, on: bool) -> None:

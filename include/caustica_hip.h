@@ -117,8 +117,19 @@ int cst_member_step(cst_session* s, int slot, int64_t* n);
 /* Advance every listed member by n_steps steps of engine.py:458-482, each from its own
  * step counter, as one batched launch sequence per step (asynchronous on the session
  * stream). Per-period max |p| over the convergence box goes to the member's peak slot
- * floor(n / spp) during steps before record_from. */
+ * floor(n / spp) during steps before record_from. The step scalars (drive amplitude and
+ * phase, record flag, period slot) are derived on the device from each member's step
+ * clock, so every step takes the same launch arguments: the sequence is captured once
+ * per member list as HIP graphs (16 steps, and 1 step for a remainder) and replayed
+ * (cst_session_set_graphs). A member whose phasors were finalized by cst_member_result
+ * is refused (CST_ERR_STATE) until cst_member_reset, set_state or set_raw. */
 int cst_advance(cst_session* s, int n_slots, const int* slots, int n_steps);
+/* Replay captured graphs (on = 1, the default) or launch every pass (0; timing with
+ * cst_session_set_profiling always launches). Clears the session's graph cache. */
+int cst_session_set_graphs(cst_session* s, int on);
+/* Graph use: *enabled (0 once a capture failed: the session then launches directly),
+ * graph launches and captures so far. */
+int cst_session_graph_stats(cst_session* s, int* enabled, int64_t* launches, int64_t* captures);
 /* Block until the session stream is idle. */
 int cst_sync(cst_session* s);
 /* *idle = 1 when every call queued on the session's stream has completed (non-blocking
@@ -131,12 +142,19 @@ int cst_session_query(cst_session* s, int* idle);
 int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out);
 /* Record-window result (engine.py:551-567): phasor of harmonic index hi as complex64
  * (interleaved re, im) over the record box, = (Kahan sum - compensation) * scale
- * evaluated in float64 on the device; pmax (may be NULL) = float32 max |p|. Synchronizes. */
+ * evaluated in float64 on the device; pmax (may be NULL) = float32 max |p|. The
+ * reduction runs once per record window, in place of the compensation: a later call for
+ * the same harmonic returns the same phasor and refuses another scale (CST_ERR_ARG), and
+ * the member cannot advance until it is reset or given a new state. Runs on the
+ * session's io stream after the member's last queued step (other members' steps may
+ * run meanwhile). Synchronizes the io stream. */
 int cst_member_result(cst_session* s, int slot, int hi, double scale, float* phasor, float* pmax);
 /* Solver state as real volumes (the state engine.py:344-371 checkpoints): p, u0, u1, u2
  * (padded shape, each may be NULL in get_state) and step n. The velocity is held on the
  * device as its z-spectrum: get_state forms u by a C2R along z, set_state by an R2C, so a
- * get/set round trip changes u by float32 transform rounding. Synchronizes. */
+ * get/set round trip changes u by float32 transform rounding. set_state and set_raw
+ * restart the record window (sums and p_max cleared: checkpoints are taken before the
+ * first record step). Synchronizes. */
 int cst_member_get_state(cst_session* s, int slot, float* p, float* u0, float* u1, float* u2,
                          int64_t* n);
 int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, const float* u0,
@@ -205,6 +223,9 @@ int cst_thermal_read(cst_thermal* t, float* theta, float* theta_max, float* dose
 int cst_thermal_heat_from_member(cst_thermal* t, cst_session* s, int slot, int n_terms,
                                  const int* harmonic_index, const double* factor, const float* w,
                                  double scale, float q_offset);
+/* Copy the session's heat source Q [W/m^3] (float32, the thermal grid) to the host, e.g.
+ * the one cst_thermal_heat_from_member formed on the device. Synchronizes. */
+int cst_thermal_read_source(cst_thermal* t, float* q);
 int cst_thermal_set_profiling(cst_thermal* t, int on);
 int cst_thermal_kernel_time(cst_thermal* t, double* total_ms, int64_t* launches);
 

@@ -7,13 +7,16 @@ Westervelt case with harmonics 1-2 under a power-law absorption model, a scaled 
 and a reference temperature away from arterial (the perfusion offset folded into Q).
 The two paths differ only in the rounding of Q (float64 |P|^2 from the float32
 components on the device, the float32 magnitude squared on the host: about 1e-7
-relative), so temperatures agree to 1e-5 K and the dose to 1e-5 relative.
+relative), so temperatures agree to 1e-5 K and the dose to 1e-5 relative. The device Q
+itself, read back, equals the oracle's float64 heat source (oracle/heating.py, pinned to
+the upstream fixture heat_power_law.npz) on the read-back phasors to 1e-6 relative.
 """
 
 import numpy as np
 import pytest
 
 from caustica_amd import workloads
+from oracle.heating import heat_source
 from caustica_amd.sensors import DeviceHeatingSource, HeatingSource
 from caustica_amd.serving.batcher import schedule_of
 from caustica_amd.solvers.kspace.native import KSpaceSession
@@ -76,3 +79,22 @@ def test_device_join_refuses_harmonics_without_model():
             DeviceHeatingSource.from_member(sess, slot, plan, req.medium, req.grid.dx, rec_steps=rec)
     finally:
         sess.close()
+
+
+@pytest.mark.parametrize("solver,y,scale", [("linear", None, 1.0), ("westervelt", 1.1, 2.5)])
+def test_device_join_q_equals_oracle_heat_source(solver, y, scale):
+    print(f"[join] Q vs oracle: {solver} y={y} scale={scale}", flush=True)
+    req, plan, sess, slot, rec = _acoustic(solver, seed=23)
+    try:
+        phasors = {h: sess.result(slot, hi, 2.0 / rec, with_pmax=False)[0].reshape(plan.rec_shape)
+                   for hi, h in enumerate(plan.harmonics)}
+        dev_q = DeviceHeatingSource.from_member(sess, slot, plan, req.medium, req.grid.dx,
+                                                rec_steps=rec, alpha_power_law_y=y).scaled(scale)
+        q = dev_q.to_host()
+    finally:
+        sess.close()
+    ref = scale * heat_source(phasors, req.medium.alpha, req.medium.rho, req.medium.c,
+                              alpha_power_law_y=y)
+    assert q.dtype == np.float32 and q.shape == ref.shape
+    assert float(ref.max()) > 0
+    np.testing.assert_allclose(q, ref, rtol=1e-6, atol=1e-6 * float(ref.max()))
