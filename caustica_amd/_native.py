@@ -65,6 +65,7 @@ EXPORTS = (
     "cst_thermal_read_source",
     "cst_thermal_set_profiling",
     "cst_thermal_kernel_time",
+    "cst_thermal_run_noalloc",
 )
 
 
@@ -126,6 +127,31 @@ class ThermalDesc(C.Structure):
     ]
 
 
+class ThermalDevDesc(C.Structure):
+    """cst_thermal_dev_desc: device pointers for cst_thermal_run_noalloc."""
+
+    _fields_ = [
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("nz", C.c_int),
+        ("dx", C.c_double),
+        ("theta", C.c_void_p),
+        ("theta_max", C.c_void_p),
+        ("dose", C.c_void_p),
+        ("k", C.c_void_p),
+        ("perf", C.c_void_p),
+        ("coeff", C.c_void_p),
+        ("q", C.c_void_p),
+        ("boundary", C.c_int),
+        ("theta_wall", C.c_float),
+        ("t_ref", C.c_float),
+        ("half_dt_min", C.c_double),
+        ("dose_on", C.c_int),
+    ]
+
+
+NOALLOC_MAX_VOXELS = 1 << 24
+
 _lib = None
 
 
@@ -174,6 +200,7 @@ def _bind(lib):
         "cst_thermal_read_source": ([v, p], i),
         "cst_thermal_set_profiling": ([v, i], i),
         "cst_thermal_kernel_time": ([v, C.POINTER(d), C.POINTER(i64)], i),
+        "cst_thermal_run_noalloc": ([C.POINTER(ThermalDevDesc), i, v, C.POINTER(i)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

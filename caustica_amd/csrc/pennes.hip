@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../include/caustica_hip.h"
@@ -24,6 +25,13 @@
 #include "thermal_state.h"
 
 using namespace cst;
+
+// Scratch of cst_thermal_run_noalloc (see the entry point): the second theta buffer of
+// the ping-pong (kNoallocMaxVoxels floats) and the non-finite flag after it.
+constexpr long kNoallocMaxVoxels = 1L << 24;  // 256^3
+constexpr size_t kScratchBytes = (size_t)kNoallocMaxVoxels * 4 + 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static_assert(kNoallocMaxVoxels == CST_NOALLOC_MAX_VOXELS, "limit stated in the header");
 
 namespace {
 
@@ -427,6 +435,100 @@ __global__ void k_f32_to_dose(const float* __restrict__ in, double* __restrict__
   if (i < n) d[i] = in ? (double)in[i] : 0.0;
 }
 
+// n explicit sub-steps ping-ponging theta between buf[cur] and buf[1 - cur] (cur is
+// updated): pairs of sub-steps as one two-step sweep (k_pennes2), a remainder as one-step
+// launches (k_pennes). The launches only: nothing is allocated or synchronized.
+int launch_substeps(ThermalArgs a, float* const buf[2], int& cur, int n_substeps, int device,
+                    hipStream_t stream, Profiler* prof) {
+  dim3 grid((a.nz + TZ - 1) / TZ, (a.ny + TY - 1) / TY, (a.nx + XCH - 1) / XCH);
+  int s = 0;
+  // pairs of sub-steps: two steps per sweep over memory (k_pennes2)
+  int ncu = 0;
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+  const long yz2 = (long)((a.nz + TZ - 1) / TZ) * ((a.ny + TY2 - 1) / TY2);
+  a.xch2 = 64;
+  while (a.xch2 > 8 && yz2 * ((a.nx + a.xch2 - 1) / a.xch2) < 2L * ncu) a.xch2 /= 2;
+  const dim3 grid2((a.nz + TZ - 1) / TZ, (a.ny + TY2 - 1) / TY2, (a.nx + a.xch2 - 1) / a.xch2);
+  auto timed = [&](auto f) {
+    if (prof) prof->run(0, stream, f);
+    else f();
+  };
+  for (; s + 2 <= n_substeps; s += 2) {
+    a.th_in = buf[cur];
+    a.th_out = buf[1 - cur];
+    timed([&] {
+      const bool hq = a.q != nullptr, dd = a.do_dose != 0;
+      const dim3 bl(TZ * TY2);
+      hipStream_t st = stream;
+      if (hq && dd) hipLaunchKernelGGL((k_pennes2<true, true>), grid2, bl, 0, st, a);
+      else if (hq) hipLaunchKernelGGL((k_pennes2<true, false>), grid2, bl, 0, st, a);
+      else if (dd) hipLaunchKernelGGL((k_pennes2<false, true>), grid2, bl, 0, st, a);
+      else hipLaunchKernelGGL((k_pennes2<false, false>), grid2, bl, 0, st, a);
+      return true;
+    });
+    HIPCHK(hipGetLastError());
+    cur = 1 - cur;
+  }
+  for (; s < n_substeps; ++s) {
+    a.th_in = buf[cur];
+    a.th_out = buf[1 - cur];
+    timed([&] {
+      const bool hq = a.q != nullptr, dd = a.do_dose != 0, full = a.nx % XCH == 0;
+      const dim3 bl(TZ * TY);
+      hipStream_t st = stream;
+      if (full) {
+        if (hq && dd) hipLaunchKernelGGL((k_pennes<true, true, true>), grid, bl, 0, st, a);
+        else if (hq) hipLaunchKernelGGL((k_pennes<true, false, true>), grid, bl, 0, st, a);
+        else if (dd) hipLaunchKernelGGL((k_pennes<false, true, true>), grid, bl, 0, st, a);
+        else hipLaunchKernelGGL((k_pennes<false, false, true>), grid, bl, 0, st, a);
+      } else {
+        if (hq && dd) hipLaunchKernelGGL((k_pennes<true, true, false>), grid, bl, 0, st, a);
+        else if (hq) hipLaunchKernelGGL((k_pennes<true, false, false>), grid, bl, 0, st, a);
+        else if (dd) hipLaunchKernelGGL((k_pennes<false, true, false>), grid, bl, 0, st, a);
+        else hipLaunchKernelGGL((k_pennes<false, false, false>), grid, bl, 0, st, a);
+      }
+      return true;
+    });
+    HIPCHK(hipGetLastError());
+    cur = 1 - cur;
+  }
+  return CST_OK;
+}
+
+// ------------------------------------------------------------ no-allocation entry point
+// Scratch of cst_thermal_run_noalloc: the second theta buffer of the ping-pong and the
+// non-finite flag, in a file-scope device array (every GPU the library runs on holds its
+// own copy until the process ends). Host code never uses the array's name as an address:
+// each device's address comes from hipGetSymbolAddress on the entry point's first call on
+// that device, and only that address reaches kernels.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(CST_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CST_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return CST_OK;
+}
+
+__global__ void k_flag_clear(unsigned* f) {
+  if (threadIdx.x == 0) f[0] = 0u;
+}
+
+__global__ void __launch_bounds__(256) k_copy_f32(const float* __restrict__ src, float* __restrict__ dst,
+                                                  long n) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
@@ -529,55 +631,7 @@ int cst_thermal_run(cst_thermal* t, int n_substeps) {
   a.t_ref = t->d.t_ref;
   a.half_dt_min = t->d.half_dt_min;
   a.do_dose = t->d.dose;
-  dim3 grid((t->nz + TZ - 1) / TZ, (t->ny + TY - 1) / TY, (t->nx + XCH - 1) / XCH);
-  int s = 0;
-  // pairs of sub-steps: two steps per sweep over memory (k_pennes2)
-  int ncu = 0;
-  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, t->device));
-  const long yz2 = (long)((t->nz + TZ - 1) / TZ) * ((t->ny + TY2 - 1) / TY2);
-  a.xch2 = 64;
-  while (a.xch2 > 8 && yz2 * ((t->nx + a.xch2 - 1) / a.xch2) < 2L * ncu) a.xch2 /= 2;
-  const dim3 grid2((t->nz + TZ - 1) / TZ, (t->ny + TY2 - 1) / TY2, (t->nx + a.xch2 - 1) / a.xch2);
-  for (; s + 2 <= n_substeps; s += 2) {
-    a.th_in = t->theta[t->cur];
-    a.th_out = t->theta[1 - t->cur];
-    t->prof.run(0, t->stream, [&] {
-      const bool hq = a.q != nullptr, dd = a.do_dose != 0;
-      const dim3 bl(TZ * TY2);
-      hipStream_t st = t->stream;
-      if (hq && dd) hipLaunchKernelGGL((k_pennes2<true, true>), grid2, bl, 0, st, a);
-      else if (hq) hipLaunchKernelGGL((k_pennes2<true, false>), grid2, bl, 0, st, a);
-      else if (dd) hipLaunchKernelGGL((k_pennes2<false, true>), grid2, bl, 0, st, a);
-      else hipLaunchKernelGGL((k_pennes2<false, false>), grid2, bl, 0, st, a);
-      return true;
-    });
-    HIPCHK(hipGetLastError());
-    t->cur = 1 - t->cur;
-  }
-  for (; s < n_substeps; ++s) {
-    a.th_in = t->theta[t->cur];
-    a.th_out = t->theta[1 - t->cur];
-    t->prof.run(0, t->stream, [&] {
-      const bool hq = a.q != nullptr, dd = a.do_dose != 0, full = a.nx % XCH == 0;
-      const dim3 bl(TZ * TY);
-      hipStream_t st = t->stream;
-      if (full) {
-        if (hq && dd) hipLaunchKernelGGL((k_pennes<true, true, true>), grid, bl, 0, st, a);
-        else if (hq) hipLaunchKernelGGL((k_pennes<true, false, true>), grid, bl, 0, st, a);
-        else if (dd) hipLaunchKernelGGL((k_pennes<false, true, true>), grid, bl, 0, st, a);
-        else hipLaunchKernelGGL((k_pennes<false, false, true>), grid, bl, 0, st, a);
-      } else {
-        if (hq && dd) hipLaunchKernelGGL((k_pennes<true, true, false>), grid, bl, 0, st, a);
-        else if (hq) hipLaunchKernelGGL((k_pennes<true, false, false>), grid, bl, 0, st, a);
-        else if (dd) hipLaunchKernelGGL((k_pennes<false, true, false>), grid, bl, 0, st, a);
-        else hipLaunchKernelGGL((k_pennes<false, false, false>), grid, bl, 0, st, a);
-      }
-      return true;
-    });
-    HIPCHK(hipGetLastError());
-    t->cur = 1 - t->cur;
-  }
-  return CST_OK;
+  return launch_substeps(a, t->theta, t->cur, n_substeps, t->device, t->stream, &t->prof);
 }
 
 int cst_thermal_status(cst_thermal* t, int* nonfinite) {
@@ -633,6 +687,59 @@ int cst_thermal_kernel_time(cst_thermal* t, double* total_ms, int64_t* launches)
   if (int r = t->prof.resolve()) return r;
   *total_ms = t->prof.ms[0];
   *launches = t->prof.count[0];
+  return CST_OK;
+}
+
+int cst_thermal_run_noalloc(const cst_thermal_dev_desc* d, int n_substeps, void* stream, int* nonfinite) {
+  if (!d) return fail(CST_ERR_ARG, "null argument");
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(CST_ERR_ARG, "invalid thermal shape");
+  const long P = (long)d->nx * d->ny * d->nz;
+  if (P > kNoallocMaxVoxels)
+    return fail(CST_ERR_ARG, "grid of " + std::to_string(P) + " voxels exceeds the no-allocation limit of " +
+                                 std::to_string(kNoallocMaxVoxels) + " voxels (use the thermal session)");
+  if (!d->theta || !d->theta_max || !d->k || !d->perf || !d->coeff || (d->dose_on && !d->dose))
+    return fail(CST_ERR_ARG, "missing device arrays");
+  if (!(d->dx > 0)) return fail(CST_ERR_ARG, "dx must be > 0");
+  if (n_substeps < 0) return fail(CST_ERR_ARG, "negative step count");
+  int device = 0;
+  HIPCHK(hipGetDevice(&device));
+  void* base = nullptr;
+  if (int r = scratch_for(device, &base)) return r;
+  float* other = static_cast<float*>(base);
+  unsigned* flag = reinterpret_cast<unsigned*>(static_cast<unsigned char*>(base) + (size_t)kNoallocMaxVoxels * 4);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(64), 0, st, flag);
+  HIPCHK(hipGetLastError());
+  ThermalArgs a{};
+  a.tmax = d->theta_max;
+  a.dose = d->dose;
+  a.k = d->k;
+  a.perf = d->perf;
+  a.coeff = d->coeff;
+  a.q = d->q;
+  a.bad = flag;
+  a.nx = d->nx;
+  a.ny = d->ny;
+  a.nz = d->nz;
+  a.inv_dx2 = (float)(1.0 / (d->dx * d->dx));
+  a.dirichlet = d->boundary;
+  a.theta_wall = d->theta_wall;
+  a.t_ref = d->t_ref;
+  a.half_dt_min = d->half_dt_min;
+  a.do_dose = d->dose_on;
+  float* const buf[2] = {d->theta, other};
+  int cur = 0;
+  if (int r = launch_substeps(a, buf, cur, n_substeps, device, st, nullptr)) return r;
+  if (cur == 1) {
+    hipLaunchKernelGGL(k_copy_f32, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, other, d->theta, P);
+    HIPCHK(hipGetLastError());
+  }
+  if (nonfinite) {
+    unsigned v = 0;
+    HIPCHK(hipMemcpyAsync(&v, flag, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *nonfinite = v ? 1 : 0;
+  }
   return CST_OK;
 }
 

@@ -229,6 +229,42 @@ int cst_thermal_read_source(cst_thermal* t, float* q);
 int cst_thermal_set_profiling(cst_thermal* t, int on);
 int cst_thermal_kernel_time(cst_thermal* t, double* total_ms, int64_t* launches);
 
+/* ------------------------------------------------ Pennes + CEM43 without allocation
+ * No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * sub-steps of cst_thermal_run on caller-owned device buffers, with the library's
+ * scratch (the second theta buffer and the non-finite flag) in a fixed device array of
+ * CST_NOALLOC_MAX_VOXELS * 4 + 256 bytes that every GPU the library runs on holds until
+ * the process ends. Nothing is allocated or freed; the call runs on the current HIP
+ * device and the given stream (NULL: the default stream). The array is shared by every
+ * call on a GPU and keeps the last call's contents: calls that overlap on one GPU race.
+ * Arrays are device pointers of C-order [nx][ny][nz] float32 (dose float64), the same
+ * quantities cst_thermal_desc describes; theta is updated in place, theta_max and dose
+ * accumulate in place (initialize theta_max to theta and the dose to the dose already
+ * taken). nonfinite (may be NULL) receives the finite-guard flag of this call and then
+ * synchronizes the stream; with NULL nothing synchronizes (capturable). Grids above
+ * CST_NOALLOC_MAX_VOXELS are refused with CST_ERR_ARG. */
+#define CST_NOALLOC_MAX_VOXELS (1L << 24)
+
+typedef struct {
+  int nx, ny, nz;
+  double dx;            /* grid spacing [m] */
+  float* theta;         /* T - T_ref, in/out */
+  float* theta_max;     /* running max of theta, in/out */
+  double* dose;         /* CEM43 [min] (float64), in/out; NULL when dose_on = 0 */
+  const float* k;       /* conductivity [W/m/K] */
+  const float* perf;    /* w_b rho_b C_b [W/m^3/K] */
+  const float* coeff;   /* dt_sub / (rho C) */
+  const float* q;       /* heat source [W/m^3] or NULL */
+  int boundary;         /* Dirichlet wall mask, as cst_thermal_desc */
+  float theta_wall;
+  float t_ref;
+  double half_dt_min;   /* 0.5 * dt_sub / 60 */
+  int dose_on;          /* accumulate CEM43 */
+} cst_thermal_dev_desc;
+
+int cst_thermal_run_noalloc(const cst_thermal_dev_desc* d, int n_substeps, void* stream, int* nonfinite);
+
 #ifdef __cplusplus
 }
 #endif
