@@ -32,6 +32,7 @@ EXPORTS = (
     "cst_session_destroy",
     "cst_session_stream",
     "cst_member_add",
+    "cst_member_add_dl",
     "cst_member_remove",
     "cst_member_reset",
     "cst_session_trim",
@@ -45,8 +46,14 @@ EXPORTS = (
     "cst_session_query",
     "cst_member_peaks",
     "cst_member_result",
+    "cst_member_result_into",
+    "cst_member_result_export",
     "cst_member_get_state",
     "cst_member_set_state",
+    "cst_member_get_state_into",
+    "cst_member_set_state_dl",
+    "cst_dl_alloc",
+    "cst_dl_copy",
     "cst_member_state_size",
     "cst_member_get_raw",
     "cst_member_set_raw",
@@ -167,6 +174,7 @@ def _bind(lib):
         "cst_session_destroy": ([v], i),
         "cst_session_stream": ([v, C.POINTER(v)], i),
         "cst_member_add": ([v, C.POINTER(MemberDesc), C.POINTER(i)], i),
+        "cst_member_add_dl": ([v, C.POINTER(MemberDesc), p, p, p, p, C.POINTER(i)], i),
         "cst_member_remove": ([v, i], i),
         "cst_member_reset": ([v, i], i),
         "cst_session_trim": ([v, C.POINTER(C.c_size_t)], i),
@@ -180,8 +188,14 @@ def _bind(lib):
         "cst_session_query": ([v, C.POINTER(i)], i),
         "cst_member_peaks": ([v, i, i, i, p], i),
         "cst_member_result": ([v, i, i, d, p, p], i),
+        "cst_member_result_into": ([v, i, i, d, p, p], i),
+        "cst_member_result_export": ([v, i, i, d, i, C.POINTER(v), C.POINTER(v)], i),
         "cst_member_get_state": ([v, i, p, p, p, p, C.POINTER(i64)], i),
         "cst_member_set_state": ([v, i, i64, p, p, p, p], i),
+        "cst_member_get_state_into": ([v, i, p, p, p, p, C.POINTER(i64)], i),
+        "cst_member_set_state_dl": ([v, i, i64, p, p, p, p], i),
+        "cst_dl_alloc": ([i, i, i, i, p, i, C.POINTER(v)], i),
+        "cst_dl_copy": ([p, p], i),
         "cst_member_state_size": ([v, C.POINTER(C.c_size_t)], i),
         "cst_member_get_raw": ([v, i, p, C.c_size_t, C.POINTER(i64)], i),
         "cst_member_set_raw": ([v, i, p, C.c_size_t, i64], i),

@@ -48,10 +48,21 @@ def _compile(src: Path, obj: Path) -> tuple[Path, int, str]:
     return src, r.returncode, out
 
 
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC"]
+
+
+def _compile_host(src: Path, obj: Path) -> tuple[Path, int, str]:
+    """Plain C++ sources (no HIP) build with the host compiler."""
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, *HOST_FLAGS, "-I", str(ROOT / "include"), "-c", str(src), "-o", str(obj)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return src, r.returncode, r.stdout + r.stderr
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> Path:
     """Compile every HIP source and link the shared library; returns its path."""
     BUILD.mkdir(exist_ok=True)
-    sources = sorted(CSRC.glob("*.hip"))
+    sources = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
     jobs = jobs or min(len(sources), max(1, min(16, os.cpu_count() or 1)))
     objs = []
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -59,7 +70,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
         for src in sources:
             obj = BUILD / (src.stem + ".o")
             objs.append(obj)
-            futs.append(ex.submit(_compile, src, obj))
+            futs.append(ex.submit(_compile_host if src.suffix == ".cpp" else _compile, src, obj))
         failed = []
         for f in futs:
             src, rc, out = f.result()

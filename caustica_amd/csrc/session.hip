@@ -19,6 +19,7 @@
 
 #include "../../include/caustica_hip.h"
 #include "common.h"
+#include "dl_abi.h"
 #include "kspace_launch.h"
 #include "kspace_types.h"
 #include "thermal_state.h"
@@ -405,14 +406,33 @@ int cst_session_stream(cst_session* s, void** stream) {
   return CST_OK;
 }
 
-int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
-  if (!s || !d || !slot_out) return fail(CST_ERR_ARG, "null argument");
+}  // extern "C"
+
+namespace {
+
+// copy kind for a checked tensor as source or destination of a device-side buffer
+hipMemcpyKind to_device(const dl::View& v) {
+  return v.where == dl::Where::Device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+}
+hipMemcpyKind from_device(const dl::View& v) {
+  return v.where == dl::Where::Device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+}
+
+int check_dl(const DLTensor* t, const char* what, DLDataType want, int64_t count, int device,
+             dl::View* out) {
+  std::string err;
+  if (!dl::check(t, what, want, count, nullptr, 0, device, out, &err)) return fail(CST_ERR_ARG, err);
+  return CST_OK;
+}
+
+// maps[0..3]: dt/rho, rho c^2 dt, absorb, 2 beta dt (data nullptr: linear)
+int member_add(cst_session* s, const cst_member_desc* d, const dl::View maps[4], int* slot_out) {
   const Shape& sh = s->s;
   if (d->n_harmonics < 1 || d->n_harmonics > kMaxHarmonics)
     return fail(CST_ERR_ARG, "n_harmonics must be 1..3");
   if (d->spp < 2 || !(d->dt > 0) || !(d->f0 > 0) || !(d->amplitude > 0) || !(d->ramp_periods > 0))
     return fail(CST_ERR_ARG, "invalid timing (spp, dt, f0, amplitude, ramp)");
-  if (!d->dt_over_rho || !d->rhoc2_dt || !d->absorb || !d->ktab || !d->sponge)
+  if (!maps[0].data || !maps[1].data || !maps[2].data || !d->ktab || !d->sponge)
     return fail(CST_ERR_ARG, "missing coefficient maps, k tables or sponge profiles");
   if (d->n_src < 0 || (d->n_src > 0 && (!d->src_idx || !d->src_phase || !d->src_scale)))
     return fail(CST_ERR_ARG, "invalid source arrays");
@@ -463,7 +483,8 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   const int64_t P = (int64_t)sh.nx * sh.ny * sh.nz;
   const int64_t H = sh.hs;  // bins per spectrum array (rows + Nyquist side plane)
   const int64_t R = (int64_t)(d->rec[1] - d->rec[0]) * (d->rec[3] - d->rec[2]) * (d->rec[5] - d->rec[4]);
-  const int nmaps = d->beta2_dt ? 4 : 3;
+  const bool nonlinear = maps[3].data != nullptr;
+  const int nmaps = nonlinear ? 4 : 3;
   const int64_t nk = 2LL * (sh.nx + sh.ny + sh.nzh);
   const int64_t nsp = sh.nz + sh.nx + sh.ny;
   const int maxp = std::max(d->max_periods, 1);
@@ -515,11 +536,11 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   md.sp = spc;
   for (int a = 0; a < 3; ++a) md.sg[a] = spc + (a + 1) * H;
   for (int a = 0; a < 3; ++a) md.su[a] = spc + (a + 4) * H;
-  float* maps = (float*)(base + o_maps);
-  md.dto_rho = maps;
-  md.rhoc2dt = maps + P;
-  md.absorb = maps + 2 * P;
-  md.beta2dt = d->beta2_dt ? maps + 3 * P : nullptr;
+  float* maps_d = (float*)(base + o_maps);
+  md.dto_rho = maps_d;
+  md.rhoc2dt = maps_d + P;
+  md.absorb = maps_d + 2 * P;
+  md.beta2dt = nonlinear ? maps_d + 3 * P : nullptr;
   md.ktab = (float*)(base + o_k);
   md.sponge = (float*)(base + o_sp);
   md.src_ptr = (int*)(base + o_ptr);
@@ -566,10 +587,10 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   };
   hipError_t e = hipMemsetAsync(base + o_state, 0, P * 4, stm);
   if (e == hipSuccess) e = hipMemsetAsync(base + o_spec, 0, 7 * H * 8, stm);
-  if (e == hipSuccess) e = up((void*)md.dto_rho, d->dt_over_rho, P * 4);
-  if (e == hipSuccess) e = up((void*)md.rhoc2dt, d->rhoc2_dt, P * 4);
-  if (e == hipSuccess) e = up((void*)md.absorb, d->absorb, P * 4);
-  if (e == hipSuccess && d->beta2_dt) e = up((void*)md.beta2dt, d->beta2_dt, P * 4);
+  // the maps come from host or device memory (DLPack views checked by the caller)
+  float* const dst_map[4] = {(float*)md.dto_rho, (float*)md.rhoc2dt, (float*)md.absorb, (float*)md.beta2dt};
+  for (int i = 0; i < nmaps && e == hipSuccess; ++i)
+    e = hipMemcpyAsync(dst_map[i], maps[i].data, P * 4, to_device(maps[i]), stm);
   if (e == hipSuccess) e = up((void*)md.ktab, d->ktab, nk * 4);
   if (e == hipSuccess) e = up((void*)md.sponge, d->sponge, nsp * 4);
   if (e == hipSuccess) e = up((void*)md.src_ptr, ptr.data(), (nl + 1) * 4);
@@ -594,10 +615,42 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
   return CST_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int cst_member_add_dl(cst_session* s, const cst_member_desc* d, const DLTensor* dt_over_rho,
+                      const DLTensor* rhoc2_dt, const DLTensor* absorb, const DLTensor* beta2_dt,
+                      int* slot) {
+  if (!s || !d || !slot) return fail(CST_ERR_ARG, "null argument");
+  const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
+  dl::View v[4];
+  const DLTensor* t[4] = {dt_over_rho, rhoc2_dt, absorb, beta2_dt};
+  const char* names[4] = {"dt_over_rho", "rhoc2_dt", "absorb", "beta2_dt"};
+  for (int i = 0; i < 4; ++i) {
+    if (i == 3 && !beta2_dt) break;
+    if (int r = check_dl(t[i], names[i], dl::kF32, P, s->device, &v[i])) return r;
+  }
+  return member_add(s, d, v, slot);
+}
+
+int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
+  if (!s || !d || !slot_out) return fail(CST_ERR_ARG, "null argument");
+  int64_t shp[3] = {s->s.nx, s->s.ny, s->s.nz};
+  const float* src[4] = {d->dt_over_rho, d->rhoc2_dt, d->absorb, d->beta2_dt};
+  DLTensor t[4];
+  for (int i = 0; i < 4; ++i) t[i] = dl::host_tensor(const_cast<float*>(src[i]), dl::kF32, shp, 3);
+  if (!d->dt_over_rho || !d->rhoc2_dt || !d->absorb)
+    return fail(CST_ERR_ARG, "missing coefficient maps, k tables or sponge profiles");
+  return cst_member_add_dl(s, d, &t[0], &t[1], &t[2], d->beta2_dt ? &t[3] : nullptr, slot_out);
+}
+
 int cst_member_remove(cst_session* s, int slot) {
   if (int r = check_slot(s, slot)) return r;
   HIPCHK(hipSetDevice(s->device));
-  HIPCHK(hipStreamSynchronize(s->stream));
+  // the member's own last step, not the whole stream: other members' steps queued after
+  // it (a chunk launched before this retirement) need not finish first
+  HIPCHK(hipEventSynchronize(s->members[slot].done));
   HIPCHK(hipStreamSynchronize(s->io));
   // keep the block for the next member of this shape (hipFree would stall every stream)
   s->pool.emplace_back(s->members[slot].block_bytes, s->members[slot].block);
@@ -675,6 +728,7 @@ int cst_member_reset(cst_session* s, int slot) {
   HIPCHK(hipMemsetAsync(m.dev.pmax, 0, m.rec_count * 4, s->stream));
   HIPCHK(hipMemsetAsync(m.dev.peaks, 0, (size_t)m.max_periods * kPeakWords * 4, s->stream));
   if (int r = set_clock(s, m, 0, INT64_MAX, 3)) return r;
+  HIPCHK(hipEventRecord(m.done, s->stream));  // cst_member_remove waits for this work too
   m.n = 0;
   m.record_from = INT64_MAX;
   m.primed = true;
@@ -685,6 +739,7 @@ int cst_member_schedule(cst_session* s, int slot, int64_t record_from) {
   if (int r = check_slot(s, slot)) return r;
   HIPCHK(hipSetDevice(s->device));
   if (int r = set_clock(s, s->members[slot], 0, record_from, 2)) return r;
+  HIPCHK(hipEventRecord(s->members[slot].done, s->stream));
   s->members[slot].record_from = record_from;
   return CST_OK;
 }
@@ -794,15 +849,21 @@ int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out)
   return CST_OK;
 }
 
-int cst_member_result(cst_session* s, int slot, int hi, double scale, float* phasor, float* pmax) {
+int cst_member_result_into(cst_session* s, int slot, int hi, double scale, DLTensor* phasor,
+                           DLTensor* pmax) {
   if (int r = check_slot(s, slot)) return r;
   Member& m = s->members[slot];
   if (hi < 0 || hi >= m.nh) return fail(CST_ERR_ARG, "harmonic index out of range");
   if (m.finalized[hi] && m.final_scale[hi] != scale)
     return fail(CST_ERR_ARG, "harmonic " + std::to_string(hi) +
                                  " was already finalized with another scale (the reduction runs once)");
-  HIPCHK(hipSetDevice(s->device));
   const int64_t R = m.rec_count;
+  dl::View vp, vm;
+  if (phasor)
+    if (int r = check_dl(phasor, "phasor", dl::kC64, R, s->device, &vp)) return r;
+  if (pmax)
+    if (int r = check_dl(pmax, "pmax", dl::kF32, R, s->device, &vm)) return r;
+  HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipStreamWaitEvent(s->io, m.done, 0));
   if (!m.finalized[hi]) {
     const int threads = 256;
@@ -816,46 +877,175 @@ int cst_member_result(cst_session* s, int slot, int hi, double scale, float* pha
     m.finalized[hi] = true;
     m.final_scale[hi] = scale;
   }
-  if (phasor) HIPCHK(hipMemcpyAsync(phasor, m.dev.comp[hi], R * 8, hipMemcpyDeviceToHost, s->io));
-  if (pmax) HIPCHK(hipMemcpyAsync(pmax, m.dev.pmax, R * 4, hipMemcpyDeviceToHost, s->io));
+  if (phasor) HIPCHK(hipMemcpyAsync(vp.data, m.dev.comp[hi], R * 8, from_device(vp), s->io));
+  if (pmax) HIPCHK(hipMemcpyAsync(vm.data, m.dev.pmax, R * 4, from_device(vm), s->io));
   HIPCHK(hipStreamSynchronize(s->io));
   return CST_OK;
 }
 
-int cst_member_get_state(cst_session* s, int slot, float* p, float* u0, float* u1, float* u2,
-                         int64_t* n) {
+int cst_member_result(cst_session* s, int slot, int hi, double scale, float* phasor, float* pmax) {
+  if (int r = check_slot(s, slot)) return r;
+  int64_t R = s->members[slot].rec_count;
+  DLTensor tp = dl::host_tensor(phasor, dl::kC64, &R, 1);
+  DLTensor tm = dl::host_tensor(pmax, dl::kF32, &R, 1);
+  return cst_member_result_into(s, slot, hi, scale, phasor ? &tp : nullptr, pmax ? &tm : nullptr);
+}
+
+}  // extern "C"
+
+namespace {
+
+void release_export(void* data, dl::Where where, int device) {
+  if (where == dl::Where::Device) {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    (void)hipFree(data);
+    (void)hipSetDevice(cur);
+  } else {
+    std::free(data);
+  }
+}
+
+// a new managed tensor of `bytes` on the host (kDLCPU) or the session's GPU (kDLROCM)
+int alloc_export(int device_type, int device, DLDataType dt, const int64_t* shape, int ndim,
+                 DLManagedTensor** out) {
+  int64_t n = 1;
+  for (int i = 0; i < ndim; ++i) n *= shape[i];
+  const size_t bytes = (size_t)std::max<int64_t>(n, 1) * (dt.bits / 8) * dt.lanes;
+  void* data = nullptr;
+  dl::Where where;
+  if (device_type == kDLROCM) {
+    where = dl::Where::Device;
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipMalloc(&data, bytes));
+  } else if (device_type == kDLCPU) {
+    where = dl::Where::Host;
+    data = std::malloc(bytes);
+    if (!data) return fail(CST_ERR_HIP, "host allocation failed");
+  } else {
+    return fail(CST_ERR_ARG, "export device type must be kDLCPU (1) or kDLROCM (10)");
+  }
+  *out = dl::make_managed(data, where, device, dt, shape, ndim, release_export);
+  if (!*out) return fail(CST_ERR_HIP, "host allocation failed");
+  return CST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cst_member_result_export(cst_session* s, int slot, int hi, double scale, int device_type,
+                             DLManagedTensor** phasor, DLManagedTensor** pmax) {
+  if (!phasor) return fail(CST_ERR_ARG, "null output");
+  *phasor = nullptr;
+  if (pmax) *pmax = nullptr;
+  if (int r = check_slot(s, slot)) return r;
+  const int* rb = s->members[slot].dev.rec;
+  const int64_t shape[3] = {rb[1] - rb[0], rb[3] - rb[2], rb[5] - rb[4]};
+  DLManagedTensor *a = nullptr, *b = nullptr;
+  if (int r = alloc_export(device_type, s->device, dl::kC64, shape, 3, &a)) return r;
+  if (pmax) {
+    if (int r = alloc_export(device_type, s->device, dl::kF32, shape, 3, &b)) {
+      a->deleter(a);
+      return r;
+    }
+  }
+  if (int r = cst_member_result_into(s, slot, hi, scale, &a->dl_tensor, b ? &b->dl_tensor : nullptr)) {
+    a->deleter(a);
+    if (b) b->deleter(b);
+    return r;
+  }
+  *phasor = a;
+  if (pmax) *pmax = b;
+  return CST_OK;
+}
+
+int cst_dl_alloc(int device_type, int device, int dtype_code, int dtype_bits, const int64_t* shape,
+                 int ndim, DLManagedTensor** out) {
+  if (!out || (ndim > 0 && !shape) || ndim < 0 || ndim > 8) return fail(CST_ERR_ARG, "invalid arguments");
+  *out = nullptr;
+  for (int i = 0; i < ndim; ++i)
+    if (shape[i] < 0) return fail(CST_ERR_ARG, "negative extent");
+  if (dtype_bits % 8 != 0 || dtype_bits < 8 || dtype_bits > 128) return fail(CST_ERR_ARG, "invalid dtype bits");
+  const DLDataType dt{(uint8_t)dtype_code, (uint8_t)dtype_bits, 1};
+  return alloc_export(device_type, device, dt, shape, ndim, out);
+}
+
+int cst_dl_copy(const DLTensor* src, DLTensor* dst) {
+  if (!src || !dst) return fail(CST_ERR_ARG, "null tensor");
+  int64_t n = 1;
+  for (int i = 0; i < src->ndim; ++i) n *= src->shape ? src->shape[i] : 0;
+  const int dev = src->device.device_type == kDLROCM ? src->device.device_id
+                  : dst->device.device_type == kDLROCM ? dst->device.device_id : 0;
+  dl::View a, b;
+  if (int r = check_dl(src, "src", src->dtype, n, dev, &a)) return r;
+  if (int r = check_dl(dst, "dst", src->dtype, n, dev, &b)) return r;
+  const size_t bytes = (size_t)n * (src->dtype.bits / 8) * src->dtype.lanes;
+  if (bytes == 0) return CST_OK;
+  if (a.where == dl::Where::Host && b.where == dl::Where::Host) {
+    std::memmove(b.data, a.data, bytes);
+    return CST_OK;
+  }
+  HIPCHK(hipSetDevice(dev));
+  HIPCHK(hipMemcpy(b.data, a.data, bytes, hipMemcpyDefault));
+  return CST_OK;
+}
+
+int cst_member_get_state_into(cst_session* s, int slot, DLTensor* p, DLTensor* u0, DLTensor* u1,
+                              DLTensor* u2, int64_t* n) {
   if (int r = check_slot(s, slot)) return r;
   Member& m = s->members[slot];
   const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
+  DLTensor* t[4] = {p, u0, u1, u2};
+  const char* names[4] = {"p", "u0", "u1", "u2"};
+  dl::View v[4];
+  for (int i = 0; i < 4; ++i)
+    if (t[i])
+      if (int r = check_dl(t[i], names[i], dl::kF32, P, s->device, &v[i])) return r;
   HIPCHK(hipSetDevice(s->device));
-  if (p) HIPCHK(hipMemcpyAsync(p, m.dev.p, P * 4, hipMemcpyDeviceToHost, s->stream));
+  if (p) HIPCHK(hipMemcpyAsync(v[0].data, m.dev.p, P * 4, from_device(v[0]), s->stream));
   // velocity: C2R of the state spectrum into the (scratch) gradient spectrum's memory
-  float* u[3] = {u0, u1, u2};
   StepArgs a = single_member(s, slot);
   for (int ax = 0; ax < 3; ++ax) {
-    if (!u[ax]) continue;
+    if (!t[ax + 1]) continue;
     if (!launch_zconv(s->s.nz / 2, a, s->tw_z, s->rt, ax, -1, s->stream))
       return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(u[ax], m.dev.sg[ax], P * 4, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(v[ax + 1].data, m.dev.sg[ax], P * 4, from_device(v[ax + 1]), s->stream));
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   if (n) *n = m.n;
   return CST_OK;
 }
 
-int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, const float* u0,
-                         const float* u1, const float* u2) {
+int cst_member_get_state(cst_session* s, int slot, float* p, float* u0, float* u1, float* u2,
+                         int64_t* n) {
+  if (!s) return fail(CST_ERR_ARG, "null session");
+  int64_t shp[3] = {s->s.nx, s->s.ny, s->s.nz};
+  float* src[4] = {p, u0, u1, u2};
+  DLTensor t[4];
+  for (int i = 0; i < 4; ++i) t[i] = dl::host_tensor(src[i], dl::kF32, shp, 3);
+  return cst_member_get_state_into(s, slot, p ? &t[0] : nullptr, u0 ? &t[1] : nullptr,
+                                   u1 ? &t[2] : nullptr, u2 ? &t[3] : nullptr, n);
+}
+
+int cst_member_set_state_dl(cst_session* s, int slot, int64_t n, const DLTensor* p, const DLTensor* u0,
+                            const DLTensor* u1, const DLTensor* u2) {
   if (int r = check_slot(s, slot)) return r;
-  if (!p || !u0 || !u1 || !u2 || n < 0) return fail(CST_ERR_ARG, "null state array or negative step");
+  if (n < 0) return fail(CST_ERR_ARG, "negative step");
   Member& m = s->members[slot];
   const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
+  const DLTensor* t[4] = {p, u0, u1, u2};
+  const char* names[4] = {"p", "u0", "u1", "u2"};
+  dl::View v[4];
+  for (int i = 0; i < 4; ++i)
+    if (int r = check_dl(t[i], names[i], dl::kF32, P, s->device, &v[i])) return r;
   HIPCHK(hipSetDevice(s->device));
-  HIPCHK(hipMemcpyAsync(m.dev.p, p, P * 4, hipMemcpyHostToDevice, s->stream));
-  const float* u[3] = {u0, u1, u2};
+  HIPCHK(hipMemcpyAsync(m.dev.p, v[0].data, P * 4, to_device(v[0]), s->stream));
   StepArgs a = single_member(s, slot);
   for (int ax = 0; ax < 3; ++ax) {
-    HIPCHK(hipMemcpyAsync(m.dev.sg[ax], u[ax], P * 4, hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(m.dev.sg[ax], v[ax + 1].data, P * 4, to_device(v[ax + 1]), s->stream));
     if (!launch_zconv(s->s.nz / 2, a, s->tw_z, s->rt, ax, +1, s->stream))
       return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
     HIPCHK(hipGetLastError());
@@ -865,6 +1055,17 @@ int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, co
   m.n = n;
   m.primed = false;
   return CST_OK;
+}
+
+int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, const float* u0,
+                         const float* u1, const float* u2) {
+  if (!s) return fail(CST_ERR_ARG, "null session");
+  if (!p || !u0 || !u1 || !u2) return fail(CST_ERR_ARG, "null state array or negative step");
+  int64_t shp[3] = {s->s.nx, s->s.ny, s->s.nz};
+  const float* src[4] = {p, u0, u1, u2};
+  DLTensor t[4];
+  for (int i = 0; i < 4; ++i) t[i] = dl::host_tensor(const_cast<float*>(src[i]), dl::kF32, shp, 3);
+  return cst_member_set_state_dl(s, slot, n, &t[0], &t[1], &t[2], &t[3]);
 }
 
 static size_t raw_state_bytes(const cst_session* s) {

@@ -136,6 +136,11 @@ class ContinuousBatcher:
     advances per chunk; a group's chunk is scaled to the same voxel-steps (small grids
     and small batches take more steps per chunk), so every chunk is a few milliseconds
     of device work and the host revisits a group only when its stream has drained.
+
+    A request that cannot be served fails alone, as an error response: planning errors
+    (a schedule that is not pinned, a shape without kernels), a footprint larger than
+    the whole budget, an allocation or upload failure at admission, a diverged field at
+    read-back. The loop always ends with one response per request.
     """
 
     def __init__(self, hbm_budget: int | None = None, max_batch: int = 16, chunk_steps: int = 16,
@@ -173,17 +178,27 @@ class ContinuousBatcher:
         """Create (and autotune) the shape group of ``plan`` ahead of the first request."""
         self._group(plan)
 
-    @staticmethod
-    def _plan(req: SolveRequest) -> MemberPlan:
-        return plan_member(req.grid, req.medium, req.source, req.spec, harmonics=req.harmonics,
+    def _plan(self, req: SolveRequest):
+        """Host planning (thread pool): the member plan, its pinned schedule and its
+        footprint, refused here if it can never fit the budget."""
+        plan = plan_member(req.grid, req.medium, req.source, req.spec, harmonics=req.harmonics,
                            record_region=req.record_region, reference_point=req.reference_point,
                            nonlinear=req.solver == "westervelt")
+        total, rec = schedule_of(plan, req.spec)
+        need = plan.device_bytes()
+        if need > self.hbm_budget:
+            raise MemoryError(f"the solve needs {need} bytes of device memory, more than the whole "
+                              f"serving budget of {self.hbm_budget} bytes")
+        return plan, total, rec, need
 
     def close(self) -> None:
         self.planners.shutdown(wait=True)
         for g in self.groups.values():
             g.session.close()
         self.groups.clear()
+
+    def _pooled(self) -> int:
+        return sum(x.session.pooled_bytes() for x in self.groups.values())
 
     def serve(self, requests: list[SolveRequest], t0: float | None = None,
               on_done=None) -> list[SolveResponse]:
@@ -194,11 +209,19 @@ class ContinuousBatcher:
         t0 = time.perf_counter() if t0 is None else t0
         now = lambda: time.perf_counter() - t0  # noqa: E731
         planned: queue.Queue = queue.Queue()
-        waiting: list = []  # (req, plan) planned but not admitted
+        waiting: list = []  # (req, (plan, total, rec, need)) planned, not admitted
         responses: list[SolveResponse] = []
         n_total = len(pending)
         nxt = 0
         in_flight_plans = 0
+
+        def fail(req, exc, admit_s=None):
+            t = now()
+            resp = SolveResponse(req.rid, None, f"{type(exc).__name__}: {exc}", req.arrival_s,
+                                 t if admit_s is None else admit_s, t, tuple(req.grid.shape))
+            responses.append(resp)
+            if on_done is not None:
+                on_done(resp)
 
         def submit_plan(req):
             fut = self.planners.submit(self._plan, req)
@@ -221,50 +244,44 @@ class ContinuousBatcher:
                 try:
                     waiting.append((req, fut.result()))
                 except Exception as exc:  # noqa: BLE001 - a bad request fails alone
-                    responses.append(SolveResponse(req.rid, None, f"{type(exc).__name__}: {exc}",
-                                                   req.arrival_s, now(), now(), tuple(req.grid.shape)))
-            # 3. groups whose stream drained: retire finished members
+                    fail(req, exc)
+            # 3. groups whose chunk has drained: collect the members whose schedule is done
+            finished = []
             for g in self.groups.values():
                 if g.busy:
                     if not g.session.idle():
                         continue
                     g.busy = False
                 for slot in [s_ for s_, lv in g.live.items() if lv.done_steps >= lv.total_steps]:
-                    lv = g.live.pop(slot)
-                    err, res = None, None
-                    try:
-                        res = finish_result(g.session, slot, lv.plan, lv.req, lv.total_steps,
-                                            lv.rec_steps, 0)
-                    except Exception as exc:  # noqa: BLE001 - a failed solve fails alone
-                        err = f"{type(exc).__name__}: {exc}"
-                    g.session.remove(slot)
-                    self.in_use -= lv.plan.device_bytes()
-                    resp = SolveResponse(lv.req.rid, res, err, lv.req.arrival_s, lv.admit_s, now(),
-                                         tuple(lv.req.grid.shape), lv.peak_batch)
-                    responses.append(resp)
-                    if on_done is not None:
-                        on_done(resp)
-            # 4. admission in arrival order, by HBM budget and free slots, into idle groups
+                    finished.append((g, g.live.pop(slot)))
+            # 4. admission in arrival order, by HBM budget and free slots; a running group
+            # takes new members too (their upload does not wait for its chunk, and they
+            # join its next one)
             still = []
-            for req, plan in waiting:
-                g = self._group(plan)
-                need = plan.device_bytes()
-                pooled = sum(x.session.pooled_bytes() for x in self.groups.values())
+            for req, (plan, total, rec, need) in waiting:
+                try:
+                    g = self._group(plan)
+                except Exception as exc:  # noqa: BLE001 - e.g. a shape without kernels
+                    fail(req, exc)
+                    continue
+                pooled = self._pooled()
                 if self.in_use + need + pooled > self.hbm_budget and pooled:
                     for x in self.groups.values():
                         if not x.busy:
                             x.session.trim()  # retired blocks give way
-                    pooled = sum(x.session.pooled_bytes() for x in self.groups.values())
-                if (not g.busy and len(g.live) < self.max_batch
-                        and self.in_use + need + pooled <= self.hbm_budget):
-                    total, rec = schedule_of(plan, req.spec)
-                    slot = g.session.add(plan, max_periods=plan.eff_max + 4)
-                    g.session.schedule(slot, total - rec)
+                    pooled = self._pooled()
+                if len(g.live) < self.max_batch and self.in_use + need + pooled <= self.hbm_budget:
+                    try:
+                        slot = g.session.add(plan, max_periods=plan.eff_max + 4)
+                        g.session.schedule(slot, total - rec)
+                    except Exception as exc:  # noqa: BLE001 - allocation / upload failure
+                        fail(req, exc)
+                        continue
                     g.live[slot] = _Live(req, plan, slot, total, rec, now())
                     self.in_use += need
                     self.peak_in_use = max(self.peak_in_use, self.in_use)
                 else:
-                    still.append((req, plan))
+                    still.append((req, (plan, total, rec, need)))
             waiting = still
             # 5. queue the next chunk of every idle group with live members; groups of
             # different shapes run on their own streams, so their chunks overlap
@@ -275,19 +292,46 @@ class ContinuousBatcher:
                 budget = self.chunk_steps * (256 ** 3)
                 k = max(1, budget // (g.voxels * len(g.live)))
                 k = min(k, min(lv.total_steps - lv.done_steps for lv in g.live.values()))
-                g.session.advance(list(g.live), k)
+                try:
+                    g.session.advance(list(g.live), k)
+                except Exception as exc:  # noqa: BLE001 - the group's members fail together
+                    for slot, lv in list(g.live.items()):
+                        g.live.pop(slot)
+                        self._retire(g, slot, lv)
+                        fail(lv.req, exc, lv.admit_s)
+                    continue
                 g.busy = True
                 launched = True
                 for lv in g.live.values():
                     lv.done_steps += k
                     lv.peak_batch = max(lv.peak_batch, len(g.live))
-            if not launched:
+            # 6. read back and retire the finished members while the chunks just queued run
+            for g, lv in finished:
+                err, res = None, None
+                try:
+                    res = finish_result(g.session, lv.slot, lv.plan, lv.req, lv.total_steps,
+                                        lv.rec_steps, 0)
+                except Exception as exc:  # noqa: BLE001 - a failed solve fails alone
+                    err = f"{type(exc).__name__}: {exc}"
+                self._retire(g, lv.slot, lv)
+                resp = SolveResponse(lv.req.rid, res, err, lv.req.arrival_s, lv.admit_s, now(),
+                                     tuple(lv.req.grid.shape), lv.peak_batch)
+                responses.append(resp)
+                if on_done is not None:
+                    on_done(resp)
+            if not launched and not finished:
                 if not any(g.busy for g in self.groups.values()) and nxt < n_total \
                         and not waiting and in_flight_plans == 0:
                     time.sleep(max(0.0, min(pending[nxt].arrival_s - now(), 0.002)))
                 else:
                     time.sleep(0.0002)
         return responses
+
+    def _retire(self, g: _Group, slot: int, lv: _Live) -> None:
+        try:
+            g.session.remove(slot)
+        finally:
+            self.in_use -= lv.plan.device_bytes()
 
 
 def serve_serial(requests: list[SolveRequest], t0: float | None = None) -> list[SolveResponse]:

@@ -3,6 +3,11 @@
 A session owns up to ``max_members`` same-shape solves on one HIP stream; a single
 solve is a one-member session. Upstream counterpart: the per-solve device state of
 ``engine.py:225-278`` (maps, operators, state p/u, source, record buffers).
+
+Volumes cross the C ABI as DLPack tensors: the coefficient maps, the solver state and
+the record-window results may be host arrays (numpy) or device tensors (any DLPack
+producer on the session's GPU, or ``caustica_amd.dlpack`` exports), borrowed for the
+call; ``result_export`` hands out new tensors on the host or the device.
 """
 
 from __future__ import annotations
@@ -12,6 +17,7 @@ import ctypes as C
 import numpy as np
 
 from caustica_amd import _native as N
+from caustica_amd import dlpack as D
 from caustica_amd.solvers.kspace.operators import MemberPlan
 
 KERNEL_KINDS = ("xgrad", "y3", "zu", "xdiv", "y1", "zp", "finalize")
@@ -56,7 +62,10 @@ class KSpaceSession:
         return int(s.value or 0)
 
     # --------------------------------------------------------------- members
-    def add(self, plan: MemberPlan, max_periods: int | None = None) -> int:
+    def add(self, plan: MemberPlan, max_periods: int | None = None, maps=None) -> int:
+        """Upload a member. ``maps`` (optional) replaces the plan's host coefficient maps
+        by DLPack tensors (dt/rho, rho c^2 dt, absorb[, 2 beta dt]) in host or device
+        memory, copied device to device when they are on the GPU."""
         if plan.padded3 != self.padded3:
             raise ValueError(f"plan shape {plan.padded3} != session shape {self.padded3}")
         d = N.MemberDesc()
@@ -78,10 +87,10 @@ class KSpaceSession:
             keep.append(b)
             return N.ptr(b)
 
-        d.dt_over_rho = arr(plan.dt_over_rho)
-        d.rhoc2_dt = arr(plan.rhoc2_dt)
-        d.absorb = arr(plan.absorb)
-        d.beta2_dt = arr(plan.beta2_dt)
+        if maps is None:
+            maps = [np.ascontiguousarray(a, np.float32) if a is not None else None
+                    for a in (plan.dt_over_rho, plan.rhoc2_dt, plan.absorb, plan.beta2_dt)]
+        maps = list(maps) + [None] * (4 - len(maps))
         d.ktab = arr(plan.ktab)
         d.sponge = arr(plan.sponge)
         d.n_src = int(plan.src_idx3.shape[0])
@@ -100,7 +109,9 @@ class KSpaceSession:
             max_periods = plan.eff_max + 4
         d.max_periods = int(max_periods)
         slot = C.c_int(-1)
-        N.check(self._lib.cst_member_add(self._h, C.byref(d), C.byref(slot)))
+        with D.borrow(maps[0]) as m0, D.borrow(maps[1]) as m1, D.borrow(maps[2]) as m2, \
+                D.borrow(maps[3]) as m3:
+            N.check(self._lib.cst_member_add_dl(self._h, C.byref(d), m0, m1, m2, m3, C.byref(slot)))
         self._plans[slot.value] = plan
         return slot.value
 
@@ -159,13 +170,28 @@ class KSpaceSession:
         R = int(np.prod([plan.rec3[1] - plan.rec3[0], plan.rec3[3] - plan.rec3[2],
                          plan.rec3[5] - plan.rec3[4]]))
         if not copy:
-            N.check(self._lib.cst_member_result(self._h, int(slot), int(hi), float(scale), None, None))
+            N.check(self._lib.cst_member_result_into(self._h, int(slot), int(hi), float(scale), None, None))
             return None, None
         ph = np.zeros(R, np.complex64)
         pm = np.zeros(R, np.float32) if with_pmax else None
-        N.check(self._lib.cst_member_result(self._h, int(slot), int(hi), float(scale), N.ptr(ph),
-                                            N.ptr(pm)))
+        self.result_into(slot, hi, scale, ph, pm)
         return ph, pm
+
+    def result_into(self, slot: int, hi: int, scale: float, phasor=None, pmax=None) -> None:
+        """Finalize harmonic ``hi`` into caller tensors (complex64 / float32 with one
+        element per record-box voxel, host or device memory; either may be None)."""
+        with D.borrow(phasor) as tp, D.borrow(pmax) as tm:
+            N.check(self._lib.cst_member_result_into(self._h, int(slot), int(hi), float(scale), tp, tm))
+
+    def result_export(self, slot: int, hi: int, scale: float, device: bool = True,
+                      with_pmax: bool = True):
+        """Finalize harmonic ``hi`` into new tensors of the record-box shape on the GPU
+        (``device=True``) or the host: (phasor, p_max or None) as ``dlpack.Exported``."""
+        a, b = C.c_void_p(), C.c_void_p()
+        N.check(self._lib.cst_member_result_export(
+            self._h, int(slot), int(hi), float(scale), D.kDLROCM if device else D.kDLCPU, C.byref(a),
+            C.byref(b) if with_pmax else None))
+        return D.Exported(a.value), (D.Exported(b.value) if with_pmax else None)
 
     def get_state(self, slot: int, with_u: bool = True):
         """([p, u0, u1, u2], n) as real volumes of the padded shape (u None unless
@@ -173,9 +199,8 @@ class KSpaceSession:
         shp = self.padded3
         arrs = [np.empty(shp, np.float32)] + [np.empty(shp, np.float32) if with_u else None
                                               for _ in range(3)]
-        n = C.c_int64()
-        N.check(self._lib.cst_member_get_state(self._h, int(slot), *[N.ptr(a) for a in arrs], C.byref(n)))
-        return arrs, int(n.value)
+        n = self.get_state_into(slot, *arrs)
+        return arrs, n
 
     def get_raw(self, slot: int):
         """(bytes of the exact device state, n): restoring it with ``set_raw`` resumes the
@@ -192,9 +217,20 @@ class KSpaceSession:
         N.check(self._lib.cst_member_set_raw(self._h, int(slot), N.ptr(b), b.nbytes, C.c_int64(int(n))))
 
     def set_state(self, slot: int, n: int, p, u0, u1, u2) -> None:
-        arrs = [np.ascontiguousarray(np.reshape(a, self.padded3), np.float32) for a in (p, u0, u1, u2)]
-        N.check(self._lib.cst_member_set_state(self._h, int(slot), C.c_int64(int(n)),
-                                               *[N.ptr(a) for a in arrs]))
+        """Restore (p, u0, u1, u2) at step n: numpy arrays of the padded shape, or any
+        DLPack tensors of nx ny nz float32 elements (host or device)."""
+        arrs = [np.ascontiguousarray(np.reshape(a, self.padded3), np.float32) if isinstance(a, np.ndarray)
+                else a for a in (p, u0, u1, u2)]
+        with D.borrow(arrs[0]) as a0, D.borrow(arrs[1]) as a1, D.borrow(arrs[2]) as a2, \
+                D.borrow(arrs[3]) as a3:
+            N.check(self._lib.cst_member_set_state_dl(self._h, int(slot), C.c_int64(int(n)), a0, a1, a2, a3))
+
+    def get_state_into(self, slot: int, p=None, u0=None, u1=None, u2=None) -> int:
+        """Copy the state into caller tensors (host or device, any may be None); returns n."""
+        n = C.c_int64()
+        with D.borrow(p) as a0, D.borrow(u0) as a1, D.borrow(u1) as a2, D.borrow(u2) as a3:
+            N.check(self._lib.cst_member_get_state_into(self._h, int(slot), a0, a1, a2, a3, C.byref(n)))
+        return int(n.value)
 
     def set_variant(self, pass_: int, variant: int) -> None:
         N.check(self._lib.cst_session_set_variant(self._h, int(pass_), int(variant)))

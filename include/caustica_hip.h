@@ -1,10 +1,16 @@
 /* caustica_hip.h -- C ABI of the MI355X (gfx950) CW k-space + Pennes/CEM43 library.
  *
- * Plain C: pointers, sizes and status codes only. Every call returns CST_OK (0) or a
- * nonzero status; cst_last_error() returns the message of the last failure on the
- * calling thread. Host arrays passed in are borrowed for the duration of the call.
- * All volumes are C-order float32 [nx][ny][nz] with z contiguous (the upstream
- * numpy layout of the padded grid).
+ * Plain C: pointers, DLPack tensors (cst_dlpack.h), sizes and status codes. Every call
+ * returns CST_OK (0) or a nonzero status; cst_last_error() returns the message of the
+ * last failure on the calling thread. All volumes are C-order float32 [nx][ny][nz] with z
+ * contiguous (the upstream numpy layout of the padded grid).
+ *
+ * Ownership. Arrays passed in -- plain pointers (host memory) or DLTensors (host memory:
+ * kDLCPU, kDLCUDAHost, kDLROCMHost; or device memory: kDLROCM on the session's GPU,
+ * compact C order) -- are borrowed for the duration of the call; the library keeps no
+ * reference. Tensors the library exports (cst_*_export, cst_dl_alloc) are
+ * DLManagedTensors owned by the caller, released by calling their deleter. The
+ * plain-pointer calls wrap their arguments in host DLTensors and call the DLPack form.
  *
  * Upstream interfaces these entry points replace (paths in the upstream caustica tree):
  *   the backend seam       src/caustica/core/backend.py:141-190, :236-270 (get_backend -> xp, fft)
@@ -21,6 +27,8 @@
 
 #include <stddef.h>
 #include <stdint.h>
+
+#include "cst_dlpack.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -91,10 +99,19 @@ typedef struct {
 } cst_member_desc;
 
 /* Upload a member; returns its slot. Allocates its device memory (state, spectra,
- * maps, record box). */
+ * maps, record box) or reuses a retired block of the same size. The upload runs on the
+ * session's upload stream and is complete when the call returns; members already in
+ * the session may be running meanwhile (admission at any step boundary). */
 int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot);
-/* Retire a member. Its device block stays with the session and is reused by the next
- * member of the same footprint (no hipFree on the serving path). */
+/* The same with the coefficient maps as DLPack tensors (float32, nx * ny * nz elements,
+ * host or device memory; beta2_dt NULL = linear); the desc's map pointers are ignored.
+ * Device maps are copied device to device: nothing crosses the bus. */
+int cst_member_add_dl(cst_session* s, const cst_member_desc* d, const DLTensor* dt_over_rho,
+                      const DLTensor* rhoc2_dt, const DLTensor* absorb, const DLTensor* beta2_dt,
+                      int* slot);
+/* Retire a member once its own queued steps are done (steps of other members queued
+ * after them may still run). Its device block stays with the session and is reused by
+ * the next member of the same footprint (no hipFree on the serving path). */
 int cst_member_remove(cst_session* s, int slot);
 /* Free the session's retired blocks; *released (may be NULL) = bytes returned. */
 int cst_session_trim(cst_session* s, size_t* released);
@@ -149,6 +166,15 @@ int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out)
  * session's io stream after the member's last queued step (other members' steps may
  * run meanwhile). Synchronizes the io stream. */
 int cst_member_result(cst_session* s, int slot, int hi, double scale, float* phasor, float* pmax);
+/* The same into caller tensors (complex64 / float32, one element per record-box voxel,
+ * host or device memory; either may be NULL). */
+int cst_member_result_into(cst_session* s, int slot, int hi, double scale, DLTensor* phasor,
+                           DLTensor* pmax);
+/* The same into new tensors of the record-box shape [rx][ry][rz] on the host
+ * (device_type kDLCPU) or the session's GPU (kDLROCM); the caller owns them. pmax may be
+ * NULL (not exported). */
+int cst_member_result_export(cst_session* s, int slot, int hi, double scale, int device_type,
+                             DLManagedTensor** phasor, DLManagedTensor** pmax);
 /* Solver state as real volumes (the state engine.py:344-371 checkpoints): p, u0, u1, u2
  * (padded shape, each may be NULL in get_state) and step n. The velocity is held on the
  * device as its z-spectrum: get_state forms u by a C2R along z, set_state by an R2C, so a
@@ -159,6 +185,12 @@ int cst_member_get_state(cst_session* s, int slot, float* p, float* u0, float* u
                          int64_t* n);
 int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, const float* u0,
                          const float* u1, const float* u2);
+/* DLPack forms of get_state / set_state: float32 tensors of nx * ny * nz elements in host
+ * or device memory (a device-resident checkpoint never crosses the bus). */
+int cst_member_get_state_into(cst_session* s, int slot, DLTensor* p, DLTensor* u0, DLTensor* u1,
+                              DLTensor* u2, int64_t* n);
+int cst_member_set_state_dl(cst_session* s, int slot, int64_t n, const DLTensor* p, const DLTensor* u0,
+                            const DLTensor* u1, const DLTensor* u2);
 /* The exact device state as bytes (p, then the three velocity spectra), for checkpoints
  * that resume bit for bit (io/checkpoint.py:127-226 with the resume contract of
  * tests/test_checkpoint.py:61-83). state_size gives the byte count for the session's
@@ -177,6 +209,14 @@ int cst_session_set_variant(cst_session* s, int pass, int variant);
  * 2 zu, 3 xdiv, 4 y (1 spectrum), 5 zp, 6 record-finalize. */
 int cst_session_set_profiling(cst_session* s, int on);
 int cst_session_kernel_time(cst_session* s, int kind, double* total_ms, int64_t* launches);
+
+/* DLPack helpers for callers without an array library of their own: a new tensor
+ * (dtype code/bits as DLDataType, one lane) on the host (kDLCPU) or GPU `device`
+ * (kDLROCM), owned by the caller; and a copy between two tensors of the same dtype and
+ * element count (host or device, either direction). */
+int cst_dl_alloc(int device_type, int device, int dtype_code, int dtype_bits, const int64_t* shape,
+                 int ndim, DLManagedTensor** out);
+int cst_dl_copy(const DLTensor* src, DLTensor* dst);
 
 /* ------------------------------------------------------------------ Pennes + CEM43
  * One thermal solve on a grid nx x ny x nz; state theta = T - T_ref (float32), CEM43 dose

@@ -1,0 +1,169 @@
+"""Host logic of the continuous batcher on the CPU, with the device session replaced.
+
+A stand-in for KSpaceSession (no GPU) records what the batcher asks of it. The batcher
+must turn every request it cannot serve into an error response and still answer every
+other request: a footprint above the whole HBM budget (it must not wait forever for
+room), a schedule that is not pinned (planning error), an allocation failure at
+admission, a diverged field at read-back. Admission into a group whose chunk is in
+flight is allowed: the new member joins the next chunk.
+"""
+
+import numpy as np
+import pytest
+
+from caustica_amd import workloads
+from caustica_amd.serving import batcher as B
+
+
+class FakeSession:
+    instances: list = []
+    fail_add_for: set = set()      # member plans (by id) whose add() raises
+    nan_for: set = set()           # plans whose peaks come back non-finite
+    busy_polls = 0                 # idle() reports busy this many times per chunk
+
+    def __init__(self, padded3, max_members=1, device=0):
+        self.padded3 = tuple(padded3)
+        self.max_members = max_members
+        self.plans = {}
+        self.next_slot = 0
+        self.log = []
+        self._busy = 0
+        FakeSession.instances.append(self)
+
+    def add(self, plan, max_periods=None):
+        if id(plan) in FakeSession.fail_add_for:
+            raise RuntimeError("hipMalloc: out of memory")
+        slot = self.next_slot
+        self.next_slot += 1
+        self.plans[slot] = plan
+        self.log.append(("add", slot, self._busy > 0))
+        return slot
+
+    def schedule(self, slot, record_from):
+        self.log.append(("schedule", slot, record_from))
+
+    def advance(self, slots, k):
+        self.log.append(("advance", tuple(slots), k))
+        self._busy = FakeSession.busy_polls
+
+    def idle(self):
+        if self._busy > 0:
+            self._busy -= 1
+            return False
+        return True
+
+    def peaks(self, slot, first, count):
+        v = np.nan if id(self.plans[slot]) in FakeSession.nan_for else 1.0
+        return np.full(count, v, np.float32)
+
+    def result(self, slot, hi, scale, with_pmax=True, copy=True):
+        n = int(np.prod(self.plans[slot].rec_shape))
+        return np.ones(n, np.complex64), (np.ones(n, np.float32) if with_pmax else None)
+
+    def remove(self, slot):
+        self.plans.pop(slot)
+        self.log.append(("remove", slot))
+
+    def pooled_bytes(self):
+        return 0
+
+    def trim(self):
+        return 0
+
+    def close(self):
+        pass
+
+
+@pytest.fixture(autouse=True)
+def fake_session(monkeypatch):
+    FakeSession.instances = []
+    FakeSession.fail_add_for = set()
+    FakeSession.nan_for = set()
+    FakeSession.busy_polls = 0
+    monkeypatch.setattr(B, "KSpaceSession", FakeSession)
+    yield
+
+
+def _requests(n, spacing=0.0):
+    rng = np.random.default_rng(3)
+    return [workloads.serving_request(i, rng, spacing * i, shape=(48, 48, 64), solver="linear")
+            for i in range(n)]
+
+
+@pytest.mark.timeout(60)
+def test_every_request_answered():
+    reqs = _requests(4)
+    b = B.ContinuousBatcher(hbm_budget=1 << 40, max_batch=2, planners=2)
+    try:
+        resp = b.serve(reqs)
+    finally:
+        b.close()
+    assert sorted(r.rid for r in resp) == [0, 1, 2, 3]
+    assert all(r.error is None for r in resp)
+    assert b.in_use == 0
+
+
+@pytest.mark.timeout(60)
+def test_request_larger_than_the_budget_fails_alone():
+    reqs = _requests(3)
+    b = B.ContinuousBatcher(hbm_budget=1 << 20, max_batch=2, planners=2)  # below one member
+    try:
+        resp = b.serve(reqs)
+    finally:
+        b.close()
+    assert len(resp) == 3
+    for r in resp:
+        assert r.result is None and "MemoryError" in r.error and "budget" in r.error
+
+
+@pytest.mark.timeout(60)
+def test_unpinned_schedule_fails_alone():
+    reqs = _requests(3)
+    reqs[1].spec = reqs[1].spec.model_copy(update={"convergence_tol": 0.01})
+    b = B.ContinuousBatcher(hbm_budget=1 << 40, max_batch=2, planners=2)
+    try:
+        resp = {r.rid: r for r in b.serve(reqs)}
+    finally:
+        b.close()
+    assert "pinned schedule" in resp[1].error
+    assert resp[0].error is None and resp[2].error is None
+
+
+@pytest.mark.timeout(60)
+def test_admission_failure_and_divergence_fail_alone(monkeypatch):
+    reqs = _requests(4)
+    orig = B.ContinuousBatcher._plan
+
+    def plan_and_mark(self, req):
+        out = orig(self, req)
+        if req.rid == 1:
+            FakeSession.fail_add_for.add(id(out[0]))
+        if req.rid == 2:
+            FakeSession.nan_for.add(id(out[0]))
+        return out
+
+    monkeypatch.setattr(B.ContinuousBatcher, "_plan", plan_and_mark)
+    b = B.ContinuousBatcher(hbm_budget=1 << 40, max_batch=4, planners=1)
+    try:
+        resp = {r.rid: r for r in b.serve(reqs)}
+    finally:
+        b.close()
+    assert "out of memory" in resp[1].error
+    assert "diverged" in resp[2].error
+    assert resp[0].error is None and resp[3].error is None
+    assert b.in_use == 0
+
+
+@pytest.mark.timeout(60)
+def test_admission_into_a_running_group():
+    FakeSession.busy_polls = 50  # each chunk stays in flight for a while
+    reqs = _requests(3, spacing=0.002)
+    b = B.ContinuousBatcher(hbm_budget=1 << 40, max_batch=4, chunk_steps=1, planners=1)
+    try:
+        resp = b.serve(reqs)
+    finally:
+        b.close()
+    assert all(r.error is None for r in resp)
+    log = FakeSession.instances[0].log
+    assert any(e[0] == "add" and e[2] for e in log), "no member was admitted while a chunk ran"
+    assert max(len(e[1]) for e in log if e[0] == "advance") >= 2

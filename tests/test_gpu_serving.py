@@ -106,3 +106,28 @@ def test_served_result_matches_fp64_oracle(plain):
     assert ref.steps_total == res.steps_total
     assert rel_l2(res.phasor, ref.phasor) <= 1e-4
     assert rel_l2(res.p_max, ref.p_max) <= 1e-4
+
+
+def test_served_benchmark_shapes_equal_alone():
+    """The serving benchmark's four grid sizes (128^3, 192^3, 256^3, 128 x 128 x 256 at
+    dx 0.375 mm, steered bowls, fresh media, linear and Westervelt with harmonics 1-2),
+    arriving close together so that groups of different shapes overlap on the device and
+    a same-shape pair shares launches: every served result bitwise the plain form's."""
+    rng = np.random.default_rng(29)
+    mix = [((128, 128, 128), "linear"), ((192, 192, 192), "linear"), ((256, 256, 256), "linear"),
+           ((128, 128, 256), "westervelt"), ((128, 128, 128), "westervelt"),
+           ((128, 128, 128), "linear")]
+    reqs = [workloads.serving_request(i, rng, 0.01 * i, shape=shape, solver=solver)
+            for i, (shape, solver) in enumerate(mix)]
+    print(f"[serving] benchmark shapes: plain form, {len(reqs)} solves", flush=True)
+    plain = _by_rid(serve_serial(reqs))
+    b = ContinuousBatcher(planners=3)
+    try:
+        print("[serving] benchmark shapes: continuous batcher", flush=True)
+        resp = b.serve(reqs)
+    finally:
+        b.close()
+    served = _by_rid(resp)
+    for rid in plain:
+        _assert_same(served[rid], plain[rid], rid)
+    assert summarize(resp)["max_batch_seen"] >= 2
