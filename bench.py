@@ -1,17 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the hip CW k-space path on one MI355X.
 
-Workload (default): the 256^3 heterogeneous linear CW solve -- layered water / skin /
+Headline workload: the 256^3 heterogeneous linear CW solve -- layered water / skin /
 fat / muscle with a seeded 2 % Gaussian random field on c and rho, dx 0.375 mm,
 16-cell sponge, bowl ROC 64 mm / aperture 64 mm at 200 kPa, CFL 0.3, 60 periods with
-the fundamental phasor from the last 2. One "step" of this benchmark is one whole
-solve (every time step of the schedule + the float64 record finalization), with the
-medium, operators and source already resident in HBM.
+the fundamental phasor from the last 2. One bench step ("--steps") is one whole solve
+(900 time steps + the float64 record finalization) with the medium, operators and
+source already resident in HBM; ``value`` is solves/s over the timed steps and
+``ms_per_step`` milliseconds per solve (``config.ms_per_time_step`` per time step).
 
-Prints ONE JSON line: solves/s, per-solve latency percentiles, the per-pass HBM
-roofline of the step's eight kernels (algorithmic bytes / measured duration, with
-durations from HIP events on the session stream), and the CPU baseline (the float32
-NumPy/scipy.fft restatement of the upstream engine, timed on this host).
+The same JSON line carries the per-pass HBM roofline of that solve (algorithmic bytes /
+duration from HIP events on the session stream), the 256 x 256 x 384 Westervelt solve,
+the Pennes/CEM43 stage heated by the resident phasor, the mixed-size serving queue
+(continuous batcher against the serial plain form; the default runs a named subset of
+the 512-request benchmark queue, ``--serving-requests 512 --serving-rates 2,8,32`` the
+whole one) and the CPU baseline (the float32 NumPy/scipy.fft restatement of the upstream
+step on this host: 3 warm-up steps, median of 20 timed steps, scipy.fft workers = 1 and
+= the host's cores).
 """
 
 from __future__ import annotations
@@ -82,39 +87,117 @@ def read_traffic(kernel: str, workload: str):
     return None
 
 
-def cpu_baseline(wl, n_steps: int, workers: int) -> dict:
-    """Median step time of the oracle's float32 restatement of the upstream engine."""
-    import numpy as np
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def host_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+def cpu_step_ms(wl, workers: int, warmup: int = 3, steps: int = 20) -> tuple[float, int]:
+    """(median step [ms], spp) of the oracle's float32 restatement of the upstream step on
+    the workload's medium and source: ``warmup`` untimed steps, then the median of ``steps``
+    individually timed ones (upstream planner/calibration.py:237-255)."""
     from oracle.kspace import KSpaceOracle, RunSpec
 
-    spec = wl.spec
     orc = KSpaceOracle(
         shape=wl.grid.shape, dx=wl.grid.dx, pml_vox=wl.grid.pml_vox, pml_edge=wl.grid.pml.edge,
         alpha=wl.medium.alpha, rho=wl.medium.rho, c=wl.medium.c, beta=wl.medium.beta,
         src_idx=wl.source.indices, src_phase=wl.source.phases, amplitude=wl.source.amplitude,
         f0=wl.source.f0, ramp_periods=wl.source.ramp_periods,
-        spec=RunSpec(**spec.model_dump()), harmonics=wl.harmonics,
+        spec=RunSpec(**wl.spec.model_dump()), harmonics=wl.harmonics,
         nonlinear=wl.solver == "westervelt", precision="fp32", workers=workers,
     )
-    orc.step(0)  # warm-up (plans, page-in)
+    n = 5 * orc.spp  # past the ramp: every source voxel is driven
+    for _ in range(warmup):
+        orc.step(n)
+        n += 1
     times = []
-    for n in range(1, n_steps + 1):
+    for _ in range(steps):
         t0 = time.perf_counter()
         orc.step(n)
         times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    steps_per_solve = wl.periods_total * orc.spp
+        n += 1
+    spp = orc.spp
     del orc
+    return statistics.median(times) * 1e3, spp
+
+
+def cpu_pennes_step_ms(shape, dx: float, warmup: int = 3, steps: int = 20) -> float:
+    """Per-step time of the oracle's float32 Pennes/CEM43 loop (upstream thermal/pennes.py
+    :608-654) on the thermal benchmark's tissue: (t(warmup + steps) - t(warmup)) / steps of
+    whole solves, which share their setup (numpy: one core)."""
+    import numpy as np
+
+    from caustica_amd import workloads
+    from oracle.pennes import PennesOracle
+
+    tm = workloads.pennes_tissue(shape, dx)
+    orc = PennesOracle(tm.k, tm.rho, tm.specific_heat, tm.perfusion, dx, precision="fp32")
+    t0 = np.full(shape, 37.0, np.float32)
+    q = np.zeros(shape, np.float32)
+    q[tuple(n // 2 for n in shape)] = 1e7
+
+    def run(k):
+        a = time.perf_counter()
+        orc.solve(t0, q, 0.05, k, dose=True)
+        return time.perf_counter() - a
+
+    short, long_ = run(warmup), run(warmup + steps)
+    return max(long_ - short, 1e-9) / steps * 1e3
+
+
+def cpu_baseline(wl, warmup: int, steps: int, legs: str, log) -> dict:
+    """The CPU baseline of the headline workload at scipy.fft workers = host cores (the
+    reported value) and workers = 1 (the upstream default); with legs == "all" also the
+    64 x 64 x 96 bowl, the Westervelt solve, the Pennes stage and the serving sizes."""
+    cores = host_cores()
+    per_solve = wl.periods_total
+    out_legs = {}
+    for w in sorted({cores, 1}, reverse=True):
+        log(f"cpu baseline: {wl.name}, workers={w}")
+        ms, spp = cpu_step_ms(wl, w, warmup, steps)
+        out_legs[f"{wl.name}_workers{w}"] = {"median_step_ms": round(ms, 2), "spp": spp,
+                                             "steps_per_solve": per_solve * spp,
+                                             "solves_per_s": 1e3 / (ms * per_solve * spp)}
+    if legs == "all":
+        from caustica_amd import workloads
+
+        extra = [("cfg0", workloads.cfg0_water_bowl()), ("cfg2", workloads.cfg2_westervelt())]
+        for tag, w2 in extra:
+            for w in sorted({cores, 1}, reverse=True):
+                log(f"cpu baseline: {w2.name}, workers={w}")
+                ms, spp = cpu_step_ms(w2, w, warmup, steps)
+                out_legs[f"{w2.name}_workers{w}"] = {
+                    "median_step_ms": round(ms, 2), "spp": spp,
+                    "steps_per_solve": w2.periods_total * spp,
+                    "solves_per_s": 1e3 / (ms * w2.periods_total * spp)}
+        log("cpu baseline: Pennes 256^3")
+        pms = cpu_pennes_step_ms((256, 256, 256), 0.375e-3, warmup, steps)
+        out_legs["pennes_256cubed"] = {"step_ms": round(pms, 2), "steps": 600,
+                                       "stage_s": round(pms * 600 / 1e3, 2), "cores": 1}
+    head = out_legs[f"{wl.name}_workers{cores}"]
     return {
-        "value": 1.0 / (med * steps_per_solve),
+        "value": head["solves_per_s"],
         "unit": "solves/s",
-        "cores": workers,
+        "cores": cores,
         "kind": "port",
-        "sample": (f"{n_steps} timed steps (+1 warm-up) of the float32 NumPy/scipy.fft restatement "
-                   f"of the upstream step on the same medium and source, scipy.fft workers={workers}; "
-                   f"median step {med * 1e3:.1f} ms x {steps_per_solve} steps per solve"),
-        "median_step_ms": med * 1e3,
+        "cpu_model": cpu_model(),
+        "sample": (f"{warmup} warm-up + {steps} individually timed steps of the float32 "
+                   f"NumPy/scipy.fft restatement of the upstream step on the same medium and "
+                   f"source (scipy.fft workers={cores}); median step {head['median_step_ms']} ms "
+                   f"x {head['steps_per_solve']} steps per solve"),
+        "legs": out_legs,
     }
 
 
@@ -244,24 +327,32 @@ def thermal_leg(sess, slot, plan, wl, rec_steps: int, n_heat: int, n_cool: int, 
     }
 
 
-def serving_leg(n_req: int, rates, seed: int, log) -> dict:
-    """Mixed-size queue (128^3, 192^3, 256^3, 128x128x256; 70 % linear / 30 % Westervelt
-    h = 1-2; steered bowls; fresh random media) replayed at Poisson arrivals, served by
-    the continuous batcher and by the serial plain form (one solve at a time through the
-    solver API). Both include host planning, upload and read-back of every request. The
-    same requests are replayed at every rate (arrival times rescaled)."""
+def serving_leg(n_req: int, rates, seed: int, serial_mode: str, log) -> dict:
+    """The mixed-size serving queue: n_req requests drawn as the benchmark queue does
+    (sizes uniform over 128^3, 192^3, 256^3, 128 x 128 x 256 at dx 0.375 mm; 70 % linear /
+    30 % Westervelt with harmonics 1-2; fresh random-field media; bowl focus steered by
+    +-10 mm lateral / +-15 mm axial with delay-and-sum phases), with Poisson arrivals at
+    each rate (the same requests, arrival times rescaled). The continuous batcher serves
+    them live. The serial plain form (one solve at a time through the solver API: host
+    planning, upload, solve, read-back) runs live at every rate (``serial_mode="live"``),
+    or once back to back, its measured per-request service times then replayed at each
+    rate's arrivals as the single FIFO server it is (``"replay"``: the plain form's
+    timeline without waiting out the idle gaps). Results are compared by digest: every
+    served result must be bitwise the plain form's."""
     import dataclasses
 
     import numpy as np
 
     from caustica_amd import workloads
-    from caustica_amd.serving import ContinuousBatcher, serve_serial, summarize
+    from caustica_amd.serving import ContinuousBatcher, replay_fifo, result_digest, serve_serial, summarize
     from caustica_amd.solvers.kspace.native import KSpaceSession
     from caustica_amd.solvers.kspace.operators import plan_member
     from caustica_amd.tune import autotune
 
+    t_gen = time.perf_counter()
     log(f"serving: generating {n_req} requests")
     base = workloads.serving_queue(n_req, rates[0], seed=seed)
+    log(f"serving: generated in {time.perf_counter() - t_gen:.1f} s")
     seen = {}
     for r in base:
         seen.setdefault(r.grid.shape, r)
@@ -270,6 +361,21 @@ def serving_leg(n_req: int, rates, seed: int, log) -> dict:
                            reference_point=r.reference_point, nonlinear=r.solver == "westervelt")
         with KSpaceSession(plan.padded3, 2) as sess:
             autotune(sess, plan)
+        del plan
+
+    from concurrent.futures import ThreadPoolExecutor
+
+    hasher = ThreadPoolExecutor(max_workers=2)  # digests off the serving loops' critical path
+
+    def keep_digest(store):
+        def cb(resp):
+            if resp.result is not None:
+                store[resp.rid] = hasher.submit(result_digest, resp.result)
+                resp.result = None  # the digest is all the comparison needs
+        return cb
+
+    def digests(store):
+        return {k: f.result() for k, f in store.items()}
 
     def pick(d):
         return {"solves_per_s": round(d["solves_per_s"], 3),
@@ -277,74 +383,93 @@ def serving_leg(n_req: int, rates, seed: int, log) -> dict:
                 "p99_ms": round(d["latency_p99_s"] * 1e3, 1),
                 "makespan_s": round(d["makespan_s"], 2), "errors": d["errors"]}
 
+    plain_digest: dict = {}
+    back = None
+    if serial_mode == "replay":
+        log(f"serving: serial plain form, {n_req} solves back to back")
+        back = serve_serial([dataclasses.replace(r, arrival_s=0.0) for r in base],
+                            on_done=keep_digest(plain_digest), wait_for_arrivals=False)
+    service = None
     runs = []
     for rate in rates:
         reqs = [dataclasses.replace(r, arrival_s=r.arrival_s * rates[0] / rate) for r in base]
-        log(f"serving at {rate} req/s: serial plain form")
-        serial = serve_serial(reqs)
+        if serial_mode == "replay":
+            serial = replay_fifo(back, {r.rid: r.arrival_s for r in reqs})
+        else:
+            log(f"serving at {rate} req/s: serial plain form")
+            plain_digest = {}
+            serial = serve_serial(reqs, on_done=keep_digest(plain_digest))
+        if service is None:
+            service = {r.rid: r.done_s - r.admit_s for r in serial if r.error is None}
         log(f"serving at {rate} req/s: continuous batcher")
+        served_digest: dict = {}
         b = ContinuousBatcher()
         try:
-            served = b.serve(reqs)
+            served = b.serve(reqs, on_done=keep_digest(served_digest))
         finally:
             b.close()
         s_ser, s_srv = summarize(serial), summarize(served)
-        # a single solve: the plain form's service time (start to done) of a 256^3 request
-        single = [r.done_s - r.admit_s for r in serial
-                  if r.shape == (256, 256, 256) and r.error is None]
+        single = [service[r.rid] for r in serial if r.shape == (256, 256, 256) and r.rid in service]
         single_ms = float(np.median(single)) * 1e3 if single else None
-        by_rid = {r.rid: r for r in serial}
-        same = all(np.array_equal(r.result.phasor, by_rid[r.rid].result.phasor)
-                   for r in served if r.error is None and by_rid[r.rid].error is None)
+        slow = [r.latency_s / service[r.rid] for r in served if r.error is None and r.rid in service]
+        pd, sd = digests(plain_digest), digests(served_digest)
+        same = bool(pd) and len(sd) == len(pd) and all(sd.get(k) == v for k, v in pd.items())
         runs.append({
             "rate_hz": rate,
-            "served": pick(s_srv) | {"max_batch_seen": s_srv["max_batch_seen"]},
-            "serial_plain": pick(s_ser),
+            "served": pick(s_srv) | {"max_batch_seen": s_srv["max_batch_seen"],
+                                     "p99_latency_over_own_plain_solve": round(float(np.percentile(slow, 99)), 2)
+                                     if slow else None},
+            "serial_plain": pick(s_ser) | {"timeline": "replayed from measured service times"
+                                           if serial_mode == "replay" else "live"},
             "speedup_vs_serial": round(s_srv["solves_per_s"] / max(s_ser["solves_per_s"], 1e-12), 2),
             "single_solve_256_ms": None if single_ms is None else round(single_ms, 1),
-            "p99_over_single": (None if single_ms is None
-                                else round(s_srv["latency_p99_s"] * 1e3 / single_ms, 2)),
-            "results_bitwise_equal": bool(same),
+            "p99_over_single_256": (None if single_ms is None
+                                    else round(s_srv["latency_p99_s"] * 1e3 / single_ms, 2)),
+            "results_bitwise_equal": same,
         })
         log(f"serving: {json.dumps(runs[-1])}")
+    hasher.shutdown(wait=True)
+    shapes = {}
+    for r in base:
+        k = "x".join(map(str, r.grid.shape))
+        shapes[k] = shapes.get(k, 0) + 1
     return {
-        "queue": {"requests": n_req, "seed": seed,
-                  "shapes": sorted({"x".join(map(str, r.grid.shape)) for r in base}),
-                  "westervelt": sum(r.solver == "westervelt" for r in base)},
+        "queue": {"requests": n_req, "of": 512, "seed": seed, "shapes": shapes,
+                  "westervelt": sum(r.solver == "westervelt" for r in base),
+                  "serial_mode": serial_mode,
+                  "mean_plain_service_ms": round(float(np.mean(list(service.values()))) * 1e3, 1)
+                  if service else None},
         "runs": runs,
     }
-
-
-def device_sync(sess):
-    sess.sync()
-    try:
-        import torch
-
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:  # noqa: BLE001 - torch is optional plumbing here
-        pass
 
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3, help="timed solves")
+    ap.add_argument("--steps", type=int, default=3, help="timed solves (bench steps)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed solves")
     ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2"])
     ap.add_argument("--batch", type=int, default=1,
                     help="same-shape solves advanced together per launch (value = all solves/s; "
                          "per-pass profile only at 1)")
-    ap.add_argument("--cpu-steps", type=int, default=5, help="timed CPU baseline steps (0: skip)")
-    ap.add_argument("--cpu-workers", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=20, help="timed CPU baseline steps (0: skip)")
+    ap.add_argument("--cpu-warmup", type=int, default=3)
+    ap.add_argument("--cpu-legs", default="headline", choices=["headline", "all"],
+                    help="CPU baseline of the headline workload only, or of every configuration")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-westervelt", dest="westervelt", action="store_false",
                     help="skip the 256x256x384 Westervelt leg")
     ap.add_argument("--thermal-steps", type=int, default=600,
                     help="Pennes steps heated by the 256^3 phasor (1/3 heating, 2/3 cooling; 0: skip)")
-    ap.add_argument("--serving-requests", type=int, default=24,
-                    help="requests of the mixed-size serving queue (0: skip the serving leg)")
+    ap.add_argument("--serving-requests", type=int, default=48,
+                    help="requests of the mixed-size serving queue (512 = the whole benchmark "
+                         "queue; 0: skip the serving leg)")
     ap.add_argument("--serving-rates", default="32,2", help="Poisson arrival rates [req/s]")
+    ap.add_argument("--serving-serial", default="auto", choices=["auto", "live", "replay"],
+                    help="serial plain form live at every rate, or its service times measured "
+                         "once and replayed (auto: live up to 64 requests)")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run the serving leg alone (prints its JSON object)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -370,6 +495,15 @@ def main() -> int:
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     get_backend("hip")
+    rates = [float(v) for v in args.serving_rates.split(",") if v]
+    serial_mode = args.serving_serial
+    if serial_mode == "auto":
+        serial_mode = "live" if args.serving_requests <= 64 else "replay"
+    if args.serving_only:
+        out = serving_leg(args.serving_requests, rates, 4, serial_mode, log)
+        print(json.dumps({"serving": out}))
+        return 0
+
     log(f"building workload {args.workload}")
     wl = {"cfg0": workloads.cfg0_water_bowl, "cfg1": workloads.cfg1_hetero_linear,
           "cfg2": workloads.cfg2_westervelt}[args.workload]()
@@ -402,21 +536,31 @@ def main() -> int:
     log(f"{wl.name}: spp {spp}, {total_steps} steps/solve; warmup {args.warmup}, timed {args.steps}")
     for _ in range(args.warmup):
         one_solve()
-    device_sync(sess)
+    sess.sync()
     lat = []
     t_start = time.perf_counter()
     for _ in range(args.steps):
         t0 = time.perf_counter()
         one_solve()
-        device_sync(sess)
+        sess.sync()
         lat.append(time.perf_counter() - t0)
-    device_sync(sess)
+    sess.sync()
     elapsed = time.perf_counter() - t_start
     value = args.batch * args.steps / elapsed
     peaks = sess.peaks(slot, 0, wl.periods_total - wl.spec.n_record_periods)
     if not np.isfinite(peaks).all():
         print("bench: non-finite field", file=sys.stderr)
         return 3
+    # the same solve with every pass launched directly (no graph replay), for the record
+    sess.set_graphs(False)
+    t0 = time.perf_counter()
+    one_solve()
+    sess.sync()
+    direct_ms = (time.perf_counter() - t0) * 1e3
+    sess.set_graphs(True)
+    graphs = {"replayed": sess.graph_stats()["enabled"],
+              "ms_per_solve_graphs": round(elapsed / args.steps * 1e3, 2),
+              "ms_per_solve_direct": round(direct_ms, 2)}
 
     passes = {}
     roofline = None
@@ -448,14 +592,13 @@ def main() -> int:
     sess.close()
     serving = None
     if args.serving_requests > 0:
-        rates = [float(v) for v in args.serving_rates.split(",") if v]
-        serving = serving_leg(args.serving_requests, rates, 4, log)
+        serving = serving_leg(args.serving_requests, rates, 4, serial_mode, log)
     cpu = None
     if args.cpu_steps > 0:
-        log(f"cpu baseline: {args.cpu_steps} steps, workers={args.cpu_workers}")
-        cpu = cpu_baseline(wl, args.cpu_steps, args.cpu_workers)
+        cpu = cpu_baseline(wl, args.cpu_warmup, args.cpu_steps, args.cpu_legs, log)
 
     lat_ms = sorted(x * 1e3 for x in lat)
+    ms_solve = elapsed / args.steps * 1e3
     out = {
         "metric": "CW solves/s + p50/p99 latency (mixed-size queue) vs serial plain; % HBM peak",
         "value": round(value, 4),
@@ -463,7 +606,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "ms_per_step": round(ms_solve, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -471,17 +614,24 @@ def main() -> int:
         "data": "synthetic (seeded layered + Gaussian-random-field medium, voxelized bowl)",
         "config": {
             "workload": wl.name,
+            "bench_step": (f"one whole CW solve of {total_steps} time steps + float64 phasor "
+                           f"finalization, maps and source resident in HBM"),
             "solver": wl.solver,
             "grid": list(wl.grid.shape),
             "periods": wl.periods_total,
             "spp": spp,
             "steps_per_solve": total_steps,
+            "ms_per_time_step": round(ms_solve / total_steps, 4),
             "harmonics": list(wl.harmonics),
             "batch": args.batch,
+            "serving_queue": (None if serving is None else
+                              f"{serving['queue']['requests']} of the 512 benchmark requests at "
+                              f"{','.join(str(r) for r in rates)} req/s"),
         },
         "variants": {str(k): v for k, v in choice.items()},
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 2),
                        "p99": round(float(np.percentile(lat_ms, 99)), 2)},
+        "graphs": graphs,
         "roofline": roofline,
         "passes": passes,
         "westervelt": westervelt,

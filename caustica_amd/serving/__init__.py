@@ -4,8 +4,11 @@ from caustica_amd.serving.batcher import (
     ContinuousBatcher,
     SolveRequest,
     SolveResponse,
+    replay_fifo,
+    result_digest,
     serve_serial,
     summarize,
 )
 
-__all__ = ["ContinuousBatcher", "SolveRequest", "SolveResponse", "serve_serial", "summarize"]
+__all__ = ["ContinuousBatcher", "SolveRequest", "SolveResponse", "replay_fifo", "result_digest",
+           "serve_serial", "summarize"]

@@ -19,8 +19,9 @@ and in the mix (tests/test_gpu_serving.py).
 
 from __future__ import annotations
 
+import dataclasses
+import hashlib
 import queue
-import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
@@ -96,6 +97,31 @@ def schedule_of(plan: MemberPlan, spec) -> tuple[int, int]:
     return settle * plan.spp + rec, rec
 
 
+_EMPTY = np.zeros(0, np.float32)
+
+
+def light_plan(plan: MemberPlan) -> MemberPlan:
+    """The plan without its host coefficient maps (uploaded already): what a live member
+    keeps for read-back and accounting (footprint, record box, schedule, nonlinearity)."""
+    return dataclasses.replace(plan, dt_over_rho=_EMPTY, rhoc2_dt=_EMPTY, absorb=_EMPTY,
+                               beta2_dt=None if plan.beta2_dt is None else _EMPTY)
+
+
+def result_digest(res: SolverResult) -> str:
+    """128-bit digest over every phasor and p_max of a result (xxh3 when the xxhash module
+    is there, BLAKE2b otherwise): equal digests = bitwise equal results."""
+    try:
+        import xxhash
+
+        h = xxhash.xxh3_128()
+    except ImportError:  # pragma: no cover
+        h = hashlib.blake2b(digest_size=16)
+    for k in sorted(res.phasors):
+        h.update(memoryview(np.ascontiguousarray(res.phasors[k])).cast("B"))
+    h.update(memoryview(np.ascontiguousarray(res.p_max)).cast("B"))
+    return h.hexdigest()
+
+
 def finish_result(sess: KSpaceSession, slot: int, plan: MemberPlan, req: SolveRequest,
                   total_steps: int, rec_steps: int, periods: int) -> SolverResult:
     """Read a finished member back into the upstream result shape (engine.py:558-586)."""
@@ -143,7 +169,7 @@ class ContinuousBatcher:
     read-back. The loop always ends with one response per request.
     """
 
-    def __init__(self, hbm_budget: int | None = None, max_batch: int = 16, chunk_steps: int = 16,
+    def __init__(self, hbm_budget: int | None = None, max_batch: int = 16, chunk_steps: int = 48,
                  planners: int = 4, device: int = 0):
         from caustica_amd import _native as N
 
@@ -158,6 +184,10 @@ class ContinuousBatcher:
         self.in_use = 0
         self.peak_in_use = 0  # high-water mark of admitted footprints (bytes)
         self.planners = ThreadPoolExecutor(max_workers=planners)
+        # host time of the serving loop by activity [s], and counts (what keeps the loop
+        # from refilling drained groups)
+        self.stats = {"admit_s": 0.0, "readback_s": 0.0, "launch_s": 0.0, "plan_wait_s": 0.0,
+                      "admitted": 0, "chunks": 0, "idle_polls": 0}
 
     def _group(self, plan: MemberPlan) -> _Group:
         g = self.groups.get(plan.padded3)
@@ -271,13 +301,17 @@ class ContinuousBatcher:
                             x.session.trim()  # retired blocks give way
                     pooled = self._pooled()
                 if len(g.live) < self.max_batch and self.in_use + need + pooled <= self.hbm_budget:
+                    ta = time.perf_counter()
                     try:
                         slot = g.session.add(plan, max_periods=plan.eff_max + 4)
                         g.session.schedule(slot, total - rec)
                     except Exception as exc:  # noqa: BLE001 - allocation / upload failure
                         fail(req, exc)
                         continue
-                    g.live[slot] = _Live(req, plan, slot, total, rec, now())
+                    finally:
+                        self.stats["admit_s"] += time.perf_counter() - ta
+                    self.stats["admitted"] += 1
+                    g.live[slot] = _Live(req, light_plan(plan), slot, total, rec, now())
                     self.in_use += need
                     self.peak_in_use = max(self.peak_in_use, self.in_use)
                 else:
@@ -292,8 +326,11 @@ class ContinuousBatcher:
                 budget = self.chunk_steps * (256 ** 3)
                 k = max(1, budget // (g.voxels * len(g.live)))
                 k = min(k, min(lv.total_steps - lv.done_steps for lv in g.live.values()))
+                tl = time.perf_counter()
                 try:
                     g.session.advance(list(g.live), k)
+                    self.stats["chunks"] += 1
+                    self.stats["launch_s"] += time.perf_counter() - tl
                 except Exception as exc:  # noqa: BLE001 - the group's members fail together
                     for slot, lv in list(g.live.items()):
                         g.live.pop(slot)
@@ -307,6 +344,7 @@ class ContinuousBatcher:
                     lv.peak_batch = max(lv.peak_batch, len(g.live))
             # 6. read back and retire the finished members while the chunks just queued run
             for g, lv in finished:
+                tr = time.perf_counter()
                 err, res = None, None
                 try:
                     res = finish_result(g.session, lv.slot, lv.plan, lv.req, lv.total_steps,
@@ -314,16 +352,20 @@ class ContinuousBatcher:
                 except Exception as exc:  # noqa: BLE001 - a failed solve fails alone
                     err = f"{type(exc).__name__}: {exc}"
                 self._retire(g, lv.slot, lv)
+                self.stats["readback_s"] += time.perf_counter() - tr
                 resp = SolveResponse(lv.req.rid, res, err, lv.req.arrival_s, lv.admit_s, now(),
                                      tuple(lv.req.grid.shape), lv.peak_batch)
                 responses.append(resp)
                 if on_done is not None:
                     on_done(resp)
             if not launched and not finished:
+                self.stats["idle_polls"] += 1
                 if not any(g.busy for g in self.groups.values()) and nxt < n_total \
                         and not waiting and in_flight_plans == 0:
                     time.sleep(max(0.0, min(pending[nxt].arrival_s - now(), 0.002)))
                 else:
+                    if in_flight_plans and not any(g.busy or g.live for g in self.groups.values()):
+                        self.stats["plan_wait_s"] += 0.0002  # the device waits for planning
                     time.sleep(0.0002)
         return responses
 
@@ -334,8 +376,12 @@ class ContinuousBatcher:
             self.in_use -= lv.plan.device_bytes()
 
 
-def serve_serial(requests: list[SolveRequest], t0: float | None = None) -> list[SolveResponse]:
-    """The plain form: one solve at a time through the solver API, in arrival order."""
+def serve_serial(requests: list[SolveRequest], t0: float | None = None, on_done=None,
+                 wait_for_arrivals: bool = True) -> list[SolveResponse]:
+    """The plain form: one solve at a time through the solver API, in arrival order.
+
+    With ``wait_for_arrivals=False`` the solves run back to back (each response's
+    admit/done times are then its service interval, the input of ``replay_fifo``)."""
     import caustica_amd.solvers as solvers
 
     pending = sorted(requests, key=lambda r: r.arrival_s)
@@ -343,7 +389,7 @@ def serve_serial(requests: list[SolveRequest], t0: float | None = None) -> list[
     out = []
     for req in pending:
         wait = req.arrival_s - (time.perf_counter() - t0)
-        if wait > 0:
+        if wait > 0 and wait_for_arrivals:
             time.sleep(wait)
         start = time.perf_counter() - t0
         err, res = None, None
@@ -354,8 +400,28 @@ def serve_serial(requests: list[SolveRequest], t0: float | None = None) -> list[
                                                 record_region=req.record_region)
         except Exception as exc:  # noqa: BLE001
             err = f"{type(exc).__name__}: {exc}"
-        out.append(SolveResponse(req.rid, res, err, req.arrival_s, start,
-                                 time.perf_counter() - t0, tuple(req.grid.shape)))
+        resp = SolveResponse(req.rid, res, err, req.arrival_s, start, time.perf_counter() - t0,
+                             tuple(req.grid.shape))
+        out.append(resp)
+        if on_done is not None:
+            on_done(resp)
+    return out
+
+
+def replay_fifo(back_to_back: list[SolveResponse], arrivals: dict) -> list[SolveResponse]:
+    """The plain form's timeline at other arrival times, from service times measured once.
+
+    One solve at a time in arrival order is a single FIFO server: solve i starts at
+    max(arrival_i, done_{i-1}) and takes its measured service time (done - admit of a
+    back-to-back run). ``arrivals`` maps rid -> arrival time [s]."""
+    order = sorted(back_to_back, key=lambda r: (arrivals[r.rid], r.rid))
+    out, free_at = [], 0.0
+    for r in order:
+        a = arrivals[r.rid]
+        start = max(a, free_at)
+        done = start + (r.done_s - r.admit_s)
+        free_at = done
+        out.append(dataclasses.replace(r, arrival_s=a, admit_s=start, done_s=done))
     return out
 
 

@@ -167,3 +167,27 @@ def test_admission_into_a_running_group():
     log = FakeSession.instances[0].log
     assert any(e[0] == "add" and e[2] for e in log), "no member was admitted while a chunk ran"
     assert max(len(e[1]) for e in log if e[0] == "advance") >= 2
+
+
+def test_fifo_replay_of_the_plain_form():
+    """A single FIFO server: start = max(arrival, previous done), done = start + service,
+    checked against an event-by-event simulation with random arrivals and services."""
+    rng = np.random.default_rng(0)
+    n = 200
+    service = rng.uniform(0.05, 0.5, n)
+    back = []
+    t = 0.0
+    for i in range(n):
+        back.append(B.SolveResponse(i, None, None, 0.0, t, t + service[i], (8, 8, 8)))
+        t += service[i]
+    arrivals = np.cumsum(rng.exponential(0.25, n))
+    out = {r.rid: r for r in B.replay_fifo(back, dict(enumerate(arrivals)))}
+    clock = 0.0
+    for i in range(n):  # arrival order is rid order here
+        clock = max(clock, arrivals[i]) + service[i]
+        assert out[i].done_s == pytest.approx(clock, abs=1e-9)
+        assert out[i].done_s - out[i].admit_s == pytest.approx(service[i], abs=1e-12)
+        assert out[i].latency_s >= service[i] - 1e-12
+    # below capacity the latency is the service time; saturated, solves/s = 1 / mean service
+    sat = B.replay_fifo(back, {i: 0.0 for i in range(n)})
+    assert B.summarize(sat)["solves_per_s"] == pytest.approx(n / service.sum(), rel=1e-9)
