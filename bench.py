@@ -351,7 +351,9 @@ def serving_leg(n_req: int, rates, seed: int, serial_mode: str, log) -> dict:
 
     t_gen = time.perf_counter()
     log(f"serving: generating {n_req} requests")
-    base = workloads.serving_queue(n_req, rates[0], seed=seed)
+    base = workloads.serving_queue(
+        n_req, rates[0], seed=seed,
+        progress=lambda i, n: log(f"serving: {i} of {n} requests built") if i % 64 == 0 else None)
     log(f"serving: generated in {time.perf_counter() - t_gen:.1f} s")
     seen = {}
     for r in base:
@@ -367,8 +369,13 @@ def serving_leg(n_req: int, rates, seed: int, serial_mode: str, log) -> dict:
 
     hasher = ThreadPoolExecutor(max_workers=2)  # digests off the serving loops' critical path
 
-    def keep_digest(store):
+    def keep_digest(store, what):
+        count = [0]
+
         def cb(resp):
+            count[0] += 1
+            if count[0] % 64 == 0:
+                log(f"serving: {what}: {count[0]} solves done")
             if resp.result is not None:
                 store[resp.rid] = hasher.submit(result_digest, resp.result)
                 resp.result = None  # the digest is all the comparison needs
@@ -388,7 +395,7 @@ def serving_leg(n_req: int, rates, seed: int, serial_mode: str, log) -> dict:
     if serial_mode == "replay":
         log(f"serving: serial plain form, {n_req} solves back to back")
         back = serve_serial([dataclasses.replace(r, arrival_s=0.0) for r in base],
-                            on_done=keep_digest(plain_digest), wait_for_arrivals=False)
+                            on_done=keep_digest(plain_digest, "serial back to back"), wait_for_arrivals=False)
     service = None
     runs = []
     for rate in rates:
@@ -398,14 +405,14 @@ def serving_leg(n_req: int, rates, seed: int, serial_mode: str, log) -> dict:
         else:
             log(f"serving at {rate} req/s: serial plain form")
             plain_digest = {}
-            serial = serve_serial(reqs, on_done=keep_digest(plain_digest))
+            serial = serve_serial(reqs, on_done=keep_digest(plain_digest, f"serial at {rate} req/s"))
         if service is None:
             service = {r.rid: r.done_s - r.admit_s for r in serial if r.error is None}
         log(f"serving at {rate} req/s: continuous batcher")
         served_digest: dict = {}
         b = ContinuousBatcher()
         try:
-            served = b.serve(reqs, on_done=keep_digest(served_digest))
+            served = b.serve(reqs, on_done=keep_digest(served_digest, f"served at {rate} req/s"))
         finally:
             b.close()
         s_ser, s_srv = summarize(serial), summarize(served)
@@ -417,6 +424,7 @@ def serving_leg(n_req: int, rates, seed: int, serial_mode: str, log) -> dict:
         runs.append({
             "rate_hz": rate,
             "served": pick(s_srv) | {"max_batch_seen": s_srv["max_batch_seen"],
+                                     "host_loop": {k: round(v, 3) for k, v in b.stats.items()},
                                      "p99_latency_over_own_plain_solve": round(float(np.percentile(slow, 99)), 2)
                                      if slow else None},
             "serial_plain": pick(s_ser) | {"timeline": "replayed from measured service times"

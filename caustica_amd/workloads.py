@@ -201,8 +201,9 @@ def serving_request(rid: int, rng: np.random.Generator, arrival_s: float, shape=
                         label=f"{solver} {'x'.join(map(str, shape))}")
 
 
-def serving_queue(n: int, rate_hz: float, seed: int = 4, shapes=None):
-    """n requests with Poisson arrivals at rate_hz (seeded exponential inter-arrival times)."""
+def serving_queue(n: int, rate_hz: float, seed: int = 4, shapes=None, progress=None):
+    """n requests with Poisson arrivals at rate_hz (seeded exponential inter-arrival times);
+    ``progress(done, n)`` (optional) is called after each request is built."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -210,6 +211,8 @@ def serving_queue(n: int, rate_hz: float, seed: int = 4, shapes=None):
         t += rng.exponential(1.0 / rate_hz)
         shape = None if shapes is None else shapes[int(rng.integers(len(shapes)))]
         out.append(serving_request(rid, rng, t, shape=shape))
+        if progress is not None:
+            progress(rid + 1, n)
     return out
 
 
