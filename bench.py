@@ -416,6 +416,9 @@ def serving_leg(n_req: int, rates, seed: int, serial_mode: str, log) -> dict:
         finally:
             b.close()
         s_ser, s_srv = summarize(serial), summarize(served)
+        errs = sorted({r.error for r in served if r.error} | {r.error for r in serial if r.error})
+        if errs:
+            log(f"serving at {rate} req/s: errors: {errs[:3]}")
         single = [service[r.rid] for r in serial if r.shape == (256, 256, 256) and r.rid in service]
         single_ms = float(np.median(single)) * 1e3 if single else None
         slow = [r.latency_s / service[r.rid] for r in served if r.error is None and r.rid in service]
@@ -434,6 +437,7 @@ def serving_leg(n_req: int, rates, seed: int, serial_mode: str, log) -> dict:
             "p99_over_single_256": (None if single_ms is None
                                     else round(s_srv["latency_p99_s"] * 1e3 / single_ms, 2)),
             "results_bitwise_equal": same,
+            "error_kinds": [e[:160] for e in errs[:3]],
         })
         log(f"serving: {json.dumps(runs[-1])}")
     hasher.shutdown(wait=True)

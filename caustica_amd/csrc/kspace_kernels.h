@@ -15,6 +15,7 @@
 // exchange and the twiddle table in LDS. Spectrum rows are padded to a multiple of 16
 // bins (128 bytes) so that every column tile of the strided passes starts on a cache line.
 #pragma once
+#include <type_traits>
 #include <utility>
 
 #include "fft_codelets.h"
@@ -39,43 +40,57 @@ constexpr int cmax(int a, int b) { return a > b ? a : b; }
 // L = A * B; stage 1: A DFTs of size B; stage 2: B DFTs of size A.
 // Strided passes: T = max(A, B) threads per line, so every thread holds one DFT per
 // stage (registers: 2 max(A, B) floats per buffer); TC columns per block, at least 16
-// (a 128-byte row segment) and the tile L x TC in LDS at most 64 KiB.
-template <int L>
+// (a 128-byte row segment) and the tile L x TC in LDS at most 64 KiB. Tile variant TV
+// (the autotuner's choice per shape, lengths >= 64): 0 that width, 1 half of it, 2 twice
+// it (within the LDS bound).
+constexpr int tile_cols(int L, int T, int TV) {
+  const int tc = cmax(1, pow2_floor(cmin(cmax(256 / T, 16), 8192 / L)));
+  if (L < 64 || TV == 0) return tc;
+  if (TV == 1) return cmax(8, tc / 2);
+  return cmin(tc * 2, pow2_floor(8192 / L));
+}
+template <int L, int TV = 0>
 struct Plan {
   static constexpr int A = plan_a(L);
   static constexpr int B = L / A;
   static constexpr int T = cmax(A, B);
-  static constexpr int TC = cmax(1, pow2_floor(cmin(cmax(256 / T, 16), 8192 / L)));
+  static constexpr int TC = tile_cols(L, T, TV);
 };
 
-// Distribution of one line FFT over T threads.
-template <int L, int T>
+// Distribution of one line FFT over T threads, stage-1 count AA (the radix plan: by
+// default the smallest divisor of L at least sqrt(L); the z passes may take its cofactor).
+template <int L, int T, int AA = Plan<L>::A>
 struct Split {
-  static constexpr int A = Plan<L>::A;
-  static constexpr int B = Plan<L>::B;
+  static constexpr int A = AA;
+  static constexpr int B = L / A;
+  static_assert(A * B == L, "radix plan must factor the length");
   static constexpr int R1 = (A + T - 1) / T;  // stage-1 DFTs per thread
   static constexpr int R2 = (B + T - 1) / T;  // stage-2 DFTs per thread
 };
 
 // z passes: complex length M = nz / 2; ZT threads per line, LPB lines per block.
-// Variant V = 0: ZT = max(A, B) (every lane busy in the larger stage); V = 1: twice the
-// threads per line (half the registers per thread); V = 2 (velocity pass only): the
-// layout of V = 0 with half the lines per block. The session picks one per shape.
+// Variant V: layout V & 3 and radix plan V >> 2. Layout 0: ZT = max(A, B) (every lane
+// busy in the larger stage); 1: twice the threads per line (half the registers per
+// thread); 2 (velocity pass only): the layout of 0 with half the lines per block. Radix
+// plan 0: A = the smallest divisor of M at least sqrt(M); 1: its cofactor (the two
+// stages swap lengths). The session picks one variant per shape.
+constexpr int zlayout(int V) { return V & 3; }
+constexpr int zradix_a(int M, int V) { return (V >> 2) ? M / plan_a(M) : plan_a(M); }
 template <int M, int V>
 struct ZPlan {
-  static constexpr int A = Plan<M>::A;
-  static constexpr int B = Plan<M>::B;
-  static constexpr int W = V == 1 ? 2 : 1;          // threads per line, in units of max(A, B)
+  static constexpr int A = zradix_a(M, V);
+  static constexpr int B = M / A;
+  static constexpr int W = zlayout(V) == 1 ? 2 : 1;  // threads per line, in units of max(A, B)
   static constexpr int ZT = cmin(64, W * cmax(A, B));
   static constexpr int LSA = M + 2;                 // natural-order line buffer (M + 1 bins)
   static constexpr int LSE = M + (M + A - 1) / A;   // exchange buffer, padded every A rows
   static constexpr int LPB0 = cmax(1, pow2_floor(cmin(W * 256 / ZT, 4480 / (LSA + LSE))));
-  static constexpr int LPB = V == 2 ? cmax(1, LPB0 / 2) : LPB0;
+  static constexpr int LPB = zlayout(V) == 2 ? cmax(1, LPB0 / 2) : LPB0;
   static constexpr int THREADS = LPB * ZT;
   // V = 2 with one-wave blocks: the natural and exchange line buffers share one LDS
   // array (two more barriers per transform, free for a single wave), which lets more
   // blocks share a CU's LDS where LDS rather than registers bounds the waves per SIMD
-  static constexpr bool ALIAS = V == 2 && THREADS <= 64;
+  static constexpr bool ALIAS = zlayout(V) == 2 && THREADS <= 64;
 };
 
 template <int DIR>
@@ -85,9 +100,10 @@ __device__ __forceinline__ float2 twmul(float2 v, float2 w) {
 
 // ---------------------------------------------------------------- stages
 // stage "in": natural-order input x[n1 + A n2]  ->  X[k2 + B k1] held as X[r][k1], k2 = t + T r
-template <int L, int T, class Load>
-__device__ __forceinline__ void in_load(int t, Load load, float2 (&V)[Split<L, T>::R1][Split<L, T>::B]) {
-  using S = Split<L, T>;
+template <int L, int T, int AA = Plan<L>::A, class Load>
+__device__ __forceinline__ void in_load(int t, Load load,
+                                        float2 (&V)[Split<L, T, AA>::R1][Split<L, T, AA>::B]) {
+  using S = Split<L, T, AA>;
 #pragma unroll
   for (int r = 0; r < S::R1; ++r) {
     const int n1 = t + T * r;
@@ -106,10 +122,10 @@ __device__ __forceinline__ void in_load(int t, Load load, float2 (&V)[Split<L, T
 // TT: the twiddle table is stored transposed, tw[k2 A + n1] = w^(n1 k2), for passes
 // whose consecutive lanes hold consecutive n1 of one line (the z passes): the natural
 // layout would have them read at stride k2 and collide on LDS banks.
-template <int L, int T, int DIR, bool TT = false, class At>
-__device__ __forceinline__ void in_compute(int t, float2 (&V)[Split<L, T>::R1][Split<L, T>::B], At at,
-                                           const float2* tw) {
-  using S = Split<L, T>;
+template <int L, int T, int DIR, bool TT = false, int AA = Plan<L>::A, class At>
+__device__ __forceinline__ void in_compute(int t, float2 (&V)[Split<L, T, AA>::R1][Split<L, T, AA>::B],
+                                           At at, const float2* tw) {
+  using S = Split<L, T, AA>;
 #pragma unroll
   for (int r = 0; r < S::R1; ++r) {
     const int n1 = t + T * r;
@@ -122,9 +138,10 @@ __device__ __forceinline__ void in_compute(int t, float2 (&V)[Split<L, T>::R1][S
   }
 }
 
-template <int L, int T, int DIR, class At>
-__device__ __forceinline__ void in_finish(int t, At at, float2 (&X)[Split<L, T>::R2][Split<L, T>::A]) {
-  using S = Split<L, T>;
+template <int L, int T, int DIR, int AA = Plan<L>::A, class At>
+__device__ __forceinline__ void in_finish(int t, At at,
+                                          float2 (&X)[Split<L, T, AA>::R2][Split<L, T, AA>::A]) {
+  using S = Split<L, T, AA>;
 #pragma unroll
   for (int r = 0; r < S::R2; ++r) {
     const int k2 = t + T * r;
@@ -289,10 +306,10 @@ __device__ __forceinline__ Col x_column(const Shape& s, int col) {
   return c;
 }
 
-template <int L, int DIR>
-__global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
+template <int L, int DIR, int TV = 0>
+__global__ void __launch_bounds__((Plan<L, TV>::TC * Plan<L, TV>::T))
     k_ypass(StepArgs a, const float2* __restrict__ twg, int which) {
-  using P = Plan<L>;
+  using P = Plan<L, TV>;
   using S = Split<L, P::T>;
   constexpr int TC = P::TC;
   constexpr int T = P::T;
@@ -376,10 +393,10 @@ __device__ __forceinline__ void y_store(int t, bool valid, gptr<float2> dst, lon
   }
 }
 
-template <int L>
-__global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
+template <int L, int TV = 0>
+__global__ void __launch_bounds__((Plan<L, TV>::TC * Plan<L, TV>::T))
     k_ygrad(StepArgs a, const float2* __restrict__ twg) {
-  using P = Plan<L>;
+  using P = Plan<L, TV>;
   using S = Split<L, P::T>;
   constexpr int TC = P::TC;
   constexpr int T = P::T;
@@ -435,10 +452,10 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
   y_store<L, T>(t, valid, g2, base, S_, X, kzd);
 }
 
-template <int L>
-__global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
+template <int L, int TV = 0>
+__global__ void __launch_bounds__((Plan<L, TV>::TC * Plan<L, TV>::T))
     k_ydiv(StepArgs a, const float2* __restrict__ twg) {
-  using P = Plan<L>;
+  using P = Plan<L, TV>;
   using S = Split<L, P::T>;
   constexpr int TC = P::TC;
   constexpr int T = P::T;
@@ -546,16 +563,18 @@ struct KxStage {
 };
 
 // Waves per SIMD requested for k_xgrad: four (128 VGPRs) where the compiler fits the
-// kernel without spilling, else no constraint.
-constexpr int xgrad_waves(int L) { return (L <= 256 && L != 240) ? 4 : 1; }
+// kernel without spilling (the default tile width), else no constraint.
+constexpr int xgrad_waves(int L, int TV) {
+  return (TV == 0 && L <= 256 && L != 144 && L != 200 && L != 240) ? 4 : 1;
+}
 
 // gradient: sg[0] = IFFT_x(kx_d i kappa / N FFT_x(sp)), sg[1] = Q = IFFT_x(i kappa / N
 // FFT_x(sp)); the y and z factors (constant along x) are applied by k_ygrad.
-template <int L>
-__global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
-    __attribute__((amdgpu_waves_per_eu(xgrad_waves(L))))
+template <int L, int TV = 0>
+__global__ void __launch_bounds__((Plan<L, TV>::TC * Plan<L, TV>::T))
+    __attribute__((amdgpu_waves_per_eu(xgrad_waves(L, TV))))
     k_xgrad(StepArgs a, const float2* __restrict__ twg) {
-  using P = Plan<L>;
+  using P = Plan<L, TV>;
   constexpr int T = P::T;
   using S = Split<L, T>;
   constexpr int TC = P::TC;
@@ -638,10 +657,10 @@ __global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
 
 // divergence: sp = IFFT_x( i kappa / N (kx_d FFT_x(sg[0]) + FFT_x(sg[1])) ), with sg[1] the
 // y/z divergence terms already combined by k_ydiv
-template <int L>
-__global__ void __launch_bounds__(Plan<L>::TC* Plan<L>::T)
+template <int L, int TV = 0>
+__global__ void __launch_bounds__((Plan<L, TV>::TC * Plan<L, TV>::T))
     k_xdiv(StepArgs a, const float2* __restrict__ twg) {
-  using P = Plan<L>;
+  using P = Plan<L, TV>;
   constexpr int T = P::T;
   using S = Split<L, T>;
   constexpr int TC = P::TC;
@@ -858,7 +877,7 @@ __device__ __forceinline__ void z_stage_load(const float2* twg, const float2* rt
   // every load is unconditional (indices clamped into range): the wait-count pass can
   // then wait for exactly the loads a consumer needs instead of for all of them
   constexpr int TH = ZStage<M, ZV>::THREADS;
-  constexpr int A = Split<M, ZPlan<M, ZV>::ZT>::A;
+  constexpr int A = ZPlan<M, ZV>::A;
   const gptr<const float2> szg = G(reinterpret_cast<const float2*>(m.sponge));
 #pragma unroll
   for (int q = 0; q < ZStage<M, ZV>::TPT; ++q) {
@@ -954,16 +973,16 @@ __device__ __forceinline__ void z_vol_load(gptr<const float2> v, int nl,
 template <int M, int ZV>
 __device__ __forceinline__ void z_c2r(int ll, int t, bool lvalid, ZLds<M, ZV>& s) {
   using Z = ZPlan<M, ZV>;
-  using S = Split<M, Z::ZT>;
+  using S = Split<M, Z::ZT, Z::A>;
   float2* la = s.a + ll * Z::LSA;
   float2* le = s.e() + ll * Z::LSE;
   float2 V[S::R1][S::B];
-  in_load<M, Z::ZT>(t, [&](int n) { return c2r_pre<M>(la, n, s.rt); }, V);
+  in_load<M, Z::ZT, Z::A>(t, [&](int n) { return c2r_pre<M>(la, n, s.rt); }, V);
   __syncthreads();  // a consumed
-  if (lvalid) in_compute<M, Z::ZT, +1, true>(t, V, LinePadAt<S::A>{le}, s.tw);
+  if (lvalid) in_compute<M, Z::ZT, +1, true, Z::A>(t, V, LinePadAt<S::A>{le}, s.tw);
   __syncthreads();
   float2 X[S::R2][S::A];
-  if (lvalid) in_finish<M, Z::ZT, +1>(t, LinePadAt<S::A>{le}, X);
+  if (lvalid) in_finish<M, Z::ZT, +1, Z::A>(t, LinePadAt<S::A>{le}, X);
   if constexpr (Z::ALIAS) __syncthreads();  // e consumed before a (the same array) is written
   if (lvalid) {
 #pragma unroll
@@ -984,16 +1003,16 @@ template <int M, int ZV>
 __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s,
                                       gptr<float2> dst_row, gptr<float2> dst_nyq) {
   using Z = ZPlan<M, ZV>;
-  using S = Split<M, Z::ZT>;
+  using S = Split<M, Z::ZT, Z::A>;
   float2* la = s.a + ll * Z::LSA;
   float2* le = s.e() + ll * Z::LSE;
   float2 V[S::R1][S::B];
-  in_load<M, Z::ZT>(t, [&](int n) { return la[n]; }, V);
+  in_load<M, Z::ZT, Z::A>(t, [&](int n) { return la[n]; }, V);
   if constexpr (Z::ALIAS) __syncthreads();  // a consumed before e (the same array) is written
-  if (lvalid) in_compute<M, Z::ZT, -1, true>(t, V, LinePadAt<S::A>{le}, s.tw);
+  if (lvalid) in_compute<M, Z::ZT, -1, true, Z::A>(t, V, LinePadAt<S::A>{le}, s.tw);
   __syncthreads();  // e written, a consumed
   float2 X[S::R2][S::A];
-  if (lvalid) in_finish<M, Z::ZT, -1>(t, LinePadAt<S::A>{le}, X);
+  if (lvalid) in_finish<M, Z::ZT, -1, Z::A>(t, LinePadAt<S::A>{le}, X);
   if constexpr (Z::ALIAS) __syncthreads();  // e consumed before a is written
   if (lvalid) {
 #pragma unroll
@@ -1017,8 +1036,11 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
 // scratch (for layout 2 with one-wave blocks, whose shared line buffer leaves the LDS
 // room for four: 150, 160, 192, 288, 300); else no constraint.
 constexpr int zu_waves(int M, int ZV) {
-  return (ZV != 1 && M != 72 && M != 80 && M != 96 && M != 100 && M != 144 && M != 200 &&
-          !(ZV == 2 && (M == 120 || M == 150 || M == 160 || M == 192 || M == 288 || M == 300))) ? 4 : 1;
+  return (zlayout(ZV) != 1 && (ZV >> 2) == 0 && M != 72 && M != 80 && M != 96 && M != 100 &&
+          M != 144 && M != 200 &&
+          !(zlayout(ZV) == 2 && (M == 120 || M == 150 || M == 160 || M == 192 || M == 288 || M == 300)))
+             ? 4
+             : 1;
 }
 
 // velocity update, one axis per block. The velocity state is kept as its z-spectrum
@@ -1354,6 +1376,39 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   }
   __syncthreads();
   z_r2c<M, ZV>(ll, t, lvalid, s, G(m.sp) + s.row[lvalid ? ll : 0], G(m.sp) + a.s.so + line0 + (lvalid ? ll : 0));
+}
+
+// ================================================================ launch dispatch
+// Tile variants of the strided passes (Plan<L, TV>): instantiated for lengths >= 64,
+// where the tile width has room to change; f receives the variant as a constant.
+template <int N, class F>
+inline void with_tile_variant(int tv, F f) {
+  if constexpr (N >= 64) {
+    if (tv == 1) return f(std::integral_constant<int, 1>{});
+    if (tv == 2) return f(std::integral_constant<int, 2>{});
+  }
+  f(std::integral_constant<int, 0>{});
+}
+
+// z-pass variants (ZPlan<M, V>): layout (V & 3) among `layouts` and radix plan (V >> 2),
+// the second radix plan only where it differs from the first (A != B).
+template <int N, int NLAYOUT, class F>
+inline void with_z_variant(int v, F f) {
+  constexpr bool two = plan_a(N) * plan_a(N) != N;
+  const int lay = (v & 3) < NLAYOUT ? (v & 3) : 0;
+  const bool r1 = two && (v >> 2) == 1;
+  if constexpr (two) {
+    if (r1) {
+      if (lay == 1) return f(std::integral_constant<int, 5>{});
+      if constexpr (NLAYOUT > 2)
+        if (lay == 2) return f(std::integral_constant<int, 6>{});
+      return f(std::integral_constant<int, 4>{});
+    }
+  }
+  if (lay == 1) return f(std::integral_constant<int, 1>{});
+  if constexpr (NLAYOUT > 2)
+    if (lay == 2) return f(std::integral_constant<int, 2>{});
+  f(std::integral_constant<int, 0>{});
 }
 
 }  // namespace cst

@@ -14,14 +14,15 @@ namespace cst {
 
 bool length_supported(int L);
 
-// which: 0 -> sp (one spectrum), 1 -> sg[0..2] (three spectra, each transformed alone)
-bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, hipStream_t st);
+// which: 0 -> sp (one spectrum), 1 -> sg[0..2] (three spectra, each transformed alone);
+// tv: tile variant of the strided passes
+bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, int tv, hipStream_t st);
 // the 3-spectrum y passes of the step (k_ygrad after xgrad, k_ydiv after zu)
-bool launch_ygrad(int L, const StepArgs& a, const float2* tw, hipStream_t st);
-bool launch_ydiv(int L, const StepArgs& a, const float2* tw, hipStream_t st);
-bool launch_xgrad(int L, const StepArgs& a, const float2* tw, hipStream_t st);
-bool launch_xdiv(int L, const StepArgs& a, const float2* tw, hipStream_t st);
-// variant: thread layout of the z passes (ZPlan<M, V>), chosen per session
+bool launch_ygrad(int L, int tv, const StepArgs& a, const float2* tw, hipStream_t st);
+bool launch_ydiv(int L, int tv, const StepArgs& a, const float2* tw, hipStream_t st);
+bool launch_xgrad(int L, int tv, const StepArgs& a, const float2* tw, hipStream_t st);
+bool launch_xdiv(int L, int tv, const StepArgs& a, const float2* tw, hipStream_t st);
+// variant: thread layout and radix plan of the z passes (ZPlan<M, V>), chosen per session
 bool launch_zu(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt,
                hipStream_t st);
 // velocity component ax between state and real volume (k_zconv): dir > 0 real -> state

@@ -4,17 +4,21 @@
 
 namespace cst {
 
-bool launch_xgrad(int L, const StepArgs& a, const float2* tw, hipStream_t st) {
+template <int N, int TV>
+static void go_xgrad(const StepArgs& a, const float2* tw, hipStream_t st) {
+  using P = Plan<N, TV>;
+  dim3 g((unsigned)strided_blocks(a.s.ny, a.s.nzh, P::TC), 1, a.nmem);
+  hipLaunchKernelGGL((k_xgrad<N, TV>), g, dim3(P::TC * P::T), 0, st, a, tw);
+}
+
+bool launch_xgrad(int L, int tv, const StepArgs& a, const float2* tw, hipStream_t st) {
   switch (L) {
-#define CST_XG(N)                                                                \
-  case N: {                                                                      \
-    using P = Plan<N>;                                                           \
-    dim3 g((unsigned)strided_blocks(a.s.ny, a.s.nzh, P::TC), 1, a.nmem);         \
-    hipLaunchKernelGGL((k_xgrad<N>), g, dim3(P::TC * P::T), 0, st, a, tw);       \
-    return true;                                                                 \
-  }
-    CST_LENGTHS(CST_XG)
-#undef CST_XG
+#define CST_X(N)                                                                   \
+  case N:                                                                          \
+    with_tile_variant<N>(tv, [&](auto c) { go_xgrad<N, decltype(c)::value>(a, tw, st); }); \
+    return true;
+    CST_LENGTHS(CST_X)
+#undef CST_X
     default:
       return false;
   }

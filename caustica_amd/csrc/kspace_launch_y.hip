@@ -4,19 +4,22 @@
 
 namespace cst {
 
-bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, hipStream_t st) {
-  const int ncomp = which == 0 ? 1 : 3;
+template <int N, int TV>
+static void go_y(int dir, const StepArgs& a, const float2* tw, int which, hipStream_t st) {
+  using P = Plan<N, TV>;
+  dim3 g(strided_blocks(a.s.nx, a.s.nzh, P::TC), which == 0 ? 1 : 3, a.nmem);
+  if (dir < 0)
+    hipLaunchKernelGGL((k_ypass<N, -1, TV>), g, dim3(P::TC * P::T), 0, st, a, tw, which);
+  else
+    hipLaunchKernelGGL((k_ypass<N, +1, TV>), g, dim3(P::TC * P::T), 0, st, a, tw, which);
+}
+
+bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, int tv, hipStream_t st) {
   switch (L) {
-#define CST_Y(N)                                                                            \
-  case N: {                                                                                 \
-    using P = Plan<N>;                                                                      \
-    dim3 g(strided_blocks(a.s.nx, a.s.nzh, P::TC), ncomp, a.nmem);                           \
-    if (dir < 0)                                                                            \
-      hipLaunchKernelGGL((k_ypass<N, -1>), g, dim3(P::TC * P::T), 0, st, a, tw, which);     \
-    else                                                                                    \
-      hipLaunchKernelGGL((k_ypass<N, +1>), g, dim3(P::TC * P::T), 0, st, a, tw, which);     \
-    return true;                                                                            \
-  }
+#define CST_Y(N)                                                                           \
+  case N:                                                                                  \
+    with_tile_variant<N>(tv, [&](auto c) { go_y<N, decltype(c)::value>(dir, a, tw, which, st); }); \
+    return true;
     CST_LENGTHS(CST_Y)
 #undef CST_Y
     default:
@@ -24,30 +27,40 @@ bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, hi
   }
 }
 
-#define CST_Y3(NAME, KERNEL)                                                          \
-  bool NAME(int L, const StepArgs& a, const float2* tw, hipStream_t st) {           \
-    switch (L) {                                                                    \
-      CST_LENGTHS(CST_Y3_CASE)                                                      \
-      default:                                                                      \
-        return false;                                                               \
-    }                                                                               \
-  }
+template <int N, int TV, bool GRAD>
+static void go_y3(const StepArgs& a, const float2* tw, hipStream_t st) {
+  using P = Plan<N, TV>;
+  dim3 g(strided_blocks(a.s.nx, a.s.nzh, P::TC), 1, a.nmem);
+  if (GRAD)
+    hipLaunchKernelGGL((k_ygrad<N, TV>), g, dim3(P::TC * P::T), 0, st, a, tw);
+  else
+    hipLaunchKernelGGL((k_ydiv<N, TV>), g, dim3(P::TC * P::T), 0, st, a, tw);
+}
 
-#define CST_Y3_CASE(N)                                                              \
-  case N: {                                                                         \
-    using P = Plan<N>;                                                              \
-    dim3 g(strided_blocks(a.s.nx, a.s.nzh, P::TC), 1, a.nmem);                        \
-    hipLaunchKernelGGL((CST_Y3_KERNEL<N>), g, dim3(P::TC * P::T), 0, st, a, tw);     \
-    return true;                                                                    \
+bool launch_ygrad(int L, int tv, const StepArgs& a, const float2* tw, hipStream_t st) {
+  switch (L) {
+#define CST_YG(N)                                                                          \
+  case N:                                                                                  \
+    with_tile_variant<N>(tv, [&](auto c) { go_y3<N, decltype(c)::value, true>(a, tw, st); }); \
+    return true;
+    CST_LENGTHS(CST_YG)
+#undef CST_YG
+    default:
+      return false;
   }
+}
 
-#define CST_Y3_KERNEL k_ygrad
-CST_Y3(launch_ygrad, k_ygrad)
-#undef CST_Y3_KERNEL
-#define CST_Y3_KERNEL k_ydiv
-CST_Y3(launch_ydiv, k_ydiv)
-#undef CST_Y3_KERNEL
-#undef CST_Y3_CASE
-#undef CST_Y3
+bool launch_ydiv(int L, int tv, const StepArgs& a, const float2* tw, hipStream_t st) {
+  switch (L) {
+#define CST_YD(N)                                                                          \
+  case N:                                                                                  \
+    with_tile_variant<N>(tv, [&](auto c) { go_y3<N, decltype(c)::value, false>(a, tw, st); }); \
+    return true;
+    CST_LENGTHS(CST_YD)
+#undef CST_YD
+    default:
+      return false;
+  }
+}
 
 }  // namespace cst

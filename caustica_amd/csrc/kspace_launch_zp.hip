@@ -19,10 +19,11 @@ static void go_zp(const StepArgs& a, const float2* tw, const float2* rt, int upd
 bool launch_zp(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt, int update,
                bool nonlinear, hipStream_t st) {
   switch (M) {
-#define CST_ZP(N)                                          \
-  case N:                                                  \
-    if (variant == 1) go_zp<N, 1>(a, tw, rt, update, nonlinear, st);  \
-    else go_zp<N, 0>(a, tw, rt, update, nonlinear, st);               \
+#define CST_ZP(N)                                                                    \
+  case N:                                                                            \
+    with_z_variant<N, 2>(variant, [&](auto c) {                                      \
+      go_zp<N, decltype(c)::value>(a, tw, rt, update, nonlinear, st);                \
+    });                                                                              \
     return true;
     CST_LENGTHS(CST_ZP)
 #undef CST_ZP

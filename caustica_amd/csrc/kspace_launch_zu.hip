@@ -16,11 +16,9 @@ static void go_zu(const StepArgs& a, const float2* tw, const float2* rt, hipStre
 bool launch_zu(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt,
                hipStream_t st) {
   switch (M) {
-#define CST_ZU(N)                                  \
-  case N:                                          \
-    if (variant == 1) go_zu<N, 1>(a, tw, rt, st);  \
-    else if (variant == 2) go_zu<N, 2>(a, tw, rt, st); \
-    else go_zu<N, 0>(a, tw, rt, st);               \
+#define CST_ZU(N)                                                                   \
+  case N:                                                                           \
+    with_z_variant<N, 3>(variant, [&](auto c) { go_zu<N, decltype(c)::value>(a, tw, rt, st); }); \
     return true;
     CST_LENGTHS(CST_ZU)
 #undef CST_ZU

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(256) k_heat_join(HeatJoin j) {
 struct GraphEntry {
   int nmem = 0;
   int ids[cst::kMaxBatch] = {};
-  int v2 = 0, v3 = 0;
+  int v[4] = {0, 0, 0, 0};  // the session's pass variants at capture
   bool nonlinear = false;
   int steps = 0;
   hipGraphExec_t exec = nullptr;
@@ -185,18 +185,20 @@ int launch_step(cst_session* s, StepArgs& a) {
   const int M = sh.nz / 2;
   hipStream_t st = s->stream;
   bool ok = true;
-  ok = ok && s->prof.run(0, st, [&] { return launch_xgrad(sh.nx, a, s->tw_x, st); });
+  ok = ok && s->prof.run(0, st, [&] { return launch_xgrad(sh.nx, s->variant[1], a, s->tw_x, st); });
   // the 3-spectrum y passes also apply the y/z derivative factors: they run for ny == 1 too
-  ok = ok && s->prof.run(1, st, [&] { return launch_ygrad(sh.ny, a, s->tw_y, st); });
+  ok = ok && s->prof.run(1, st, [&] { return launch_ygrad(sh.ny, s->variant[0], a, s->tw_y, st); });
   ok = ok && s->prof.run(2, st, [&] { return launch_zu(M, s->variant[2], a, s->tw_z, s->rt, st); });
-  ok = ok && s->prof.run(1, st, [&] { return launch_ydiv(sh.ny, a, s->tw_y, st); });
-  ok = ok && s->prof.run(3, st, [&] { return launch_xdiv(sh.nx, a, s->tw_x, st); });
-  if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 0, st); });
+  ok = ok && s->prof.run(1, st, [&] { return launch_ydiv(sh.ny, s->variant[0], a, s->tw_y, st); });
+  ok = ok && s->prof.run(3, st, [&] { return launch_xdiv(sh.nx, s->variant[1], a, s->tw_x, st); });
+  if (sh.ny > 1)
+    ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 0, s->variant[0], st); });
   const bool nonlinear = batch_nonlinear(s, a);
   ok = ok && s->prof.run(5, st, [&] {
     return launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 1, nonlinear, st);
   });
-  if (sh.ny > 1) ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 0, st); });
+  if (sh.ny > 1)
+    ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 0, s->variant[0], st); });
   if (!ok) return fail(CST_ERR_UNSUPPORTED, "no kernel instantiated for this shape");
   HIPCHK(hipGetLastError());
   return CST_OK;
@@ -239,7 +241,7 @@ constexpr size_t kGraphCache = 48;
 hipGraphExec_t graph_for(cst_session* s, StepArgs& a, int steps) {
   const bool nl = batch_nonlinear(s, a);
   for (auto& g : s->gcache) {
-    if (g.nmem != a.nmem || g.steps != steps || g.v2 != s->variant[2] || g.v3 != s->variant[3] ||
+    if (g.nmem != a.nmem || g.steps != steps || !std::equal(s->variant, s->variant + 4, g.v) ||
         g.nonlinear != nl)
       continue;
     if (!std::equal(a.ids, a.ids + a.nmem, g.ids)) continue;
@@ -274,8 +276,7 @@ hipGraphExec_t graph_for(cst_session* s, StepArgs& a, int steps) {
   GraphEntry g;
   g.nmem = a.nmem;
   std::copy(a.ids, a.ids + a.nmem, g.ids);
-  g.v2 = s->variant[2];
-  g.v3 = s->variant[3];
+  std::copy(s->variant, s->variant + 4, g.v);
   g.nonlinear = nl;
   g.steps = steps;
   g.exec = exec;
@@ -291,7 +292,7 @@ int prime(cst_session* s, int slot) {
   const int M = s->s.nz / 2;
   if (!launch_zp(M, s->variant[3], a, s->tw_z, s->rt, 0, false, s->stream))
     return fail(CST_ERR_UNSUPPORTED, "no kernel for nz");
-  if (s->s.ny > 1 && !launch_y(s->s.ny, -1, a, s->tw_y, 0, s->stream))
+  if (s->s.ny > 1 && !launch_y(s->s.ny, -1, a, s->tw_y, 0, s->variant[0], s->stream))
     return fail(CST_ERR_UNSUPPORTED, "no kernel for ny");
   HIPCHK(hipGetLastError());
   s->members[slot].primed = true;
@@ -311,7 +312,7 @@ __global__ void __launch_bounds__(256) k_diag_copy(const float4* __restrict__ sr
 
 extern "C" {
 
-int cst_version(void) { return 102; }
+int cst_version(void) { return 103; }
 
 int cst_last_error(char* buf, size_t n) { return copy_last_error(buf, n); }
 
@@ -1114,9 +1115,14 @@ int cst_member_set_raw(cst_session* s, int slot, const void* buf, size_t bytes, 
 
 int cst_session_set_variant(cst_session* s, int pass, int variant) {
   if (!s) return fail(CST_ERR_ARG, "null session");
-  if (pass < 0 || pass > 3 || variant < 0 || variant > (pass == 2 ? 2 : 1))
-    return fail(CST_ERR_ARG, "pass must be 0..3 and variant 0..1 (0..2 for the velocity pass)");
-  s->variant[pass] = variant;
+  bool ok = false;
+  if (pass == 0 || pass == 1) ok = variant >= 0 && variant <= 2;
+  if (pass == 2) ok = (variant >= 0 && variant <= 2) || (variant >= 4 && variant <= 6);
+  if (pass == 3) ok = variant == 0 || variant == 1 || variant == 4 || variant == 5;
+  if (!ok)
+    return fail(CST_ERR_ARG, "pass 0 (y) / 1 (x): tile variant 0..2; pass 2 (zu): layout 0..2 + 4 x radix "
+                             "plan 0..1; pass 3 (zp): layout 0..1 + 4 x radix plan 0..1");
+  s->variant[pass] = variant;  // cached graphs are keyed by the variants they captured
   return CST_OK;
 }
 

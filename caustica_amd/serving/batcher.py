@@ -287,6 +287,10 @@ class ContinuousBatcher:
             # 4. admission in arrival order, by HBM budget and free slots; a running group
             # takes new members too (their upload does not wait for its chunk, and they
             # join its next one)
+            # members collected above still hold their session slots until step 6
+            retiring: dict = {}
+            for g_, _lv in finished:
+                retiring[id(g_)] = retiring.get(id(g_), 0) + 1
             still = []
             for req, (plan, total, rec, need) in waiting:
                 try:
@@ -300,7 +304,8 @@ class ContinuousBatcher:
                         if not x.busy:
                             x.session.trim()  # retired blocks give way
                     pooled = self._pooled()
-                if len(g.live) < self.max_batch and self.in_use + need + pooled <= self.hbm_budget:
+                occupied = len(g.live) + retiring.get(id(g), 0)
+                if occupied < self.max_batch and self.in_use + need + pooled <= self.hbm_budget:
                     ta = time.perf_counter()
                     try:
                         slot = g.session.add(plan, max_periods=plan.eff_max + 4)

@@ -199,9 +199,11 @@ int cst_member_state_size(cst_session* s, size_t* bytes);
 int cst_member_get_raw(cst_session* s, int slot, void* buf, size_t bytes, int64_t* n);
 int cst_member_set_raw(cst_session* s, int slot, const void* buf, size_t bytes, int64_t n);
 
-/* Select the thread-layout variant of a tunable pass (2 zu: 0, 1 or 2; 3 zp: 0 or 1;
- * passes 0 and 1 accept 0 or 1 and have one layout) for every later step of the session;
- * the autotuner times the variants and keeps the fastest. */
+/* Select the variant of a tunable pass for every later step of the session; the
+ * autotuner times them per shape and keeps the fastest. Pass 0 (the y passes) and 1 (the
+ * x passes): column-tile width 0 (default), 1 (half), 2 (twice; lengths >= 64). Pass 2
+ * (zu) and 3 (zp): thread layout (zu 0..2, zp 0..1) + 4 x radix plan (0: stage-1 count
+ * the smallest divisor of nz/2 at least sqrt(nz/2), 1: its cofactor). */
 int cst_session_set_variant(cst_session* s, int pass, int variant);
 
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on the session

@@ -33,6 +33,8 @@ class FakeSession:
     def add(self, plan, max_periods=None):
         if id(plan) in FakeSession.fail_add_for:
             raise RuntimeError("hipMalloc: out of memory")
+        if len(self.plans) >= self.max_members:  # as cst_member_add: every slot is taken
+            raise RuntimeError("session is full")
         slot = self.next_slot
         self.next_slot += 1
         self.plans[slot] = plan
@@ -191,3 +193,18 @@ def test_fifo_replay_of_the_plain_form():
     # below capacity the latency is the service time; saturated, solves/s = 1 / mean service
     sat = B.replay_fifo(back, {i: 0.0 for i in range(n)})
     assert B.summarize(sat)["solves_per_s"] == pytest.approx(n / service.sum(), rel=1e-9)
+
+
+@pytest.mark.timeout(60)
+def test_slots_of_retiring_members_are_not_reused_early():
+    """A member whose schedule is done holds its session slot until it is read back and
+    removed: admissions in the same loop turn must not count that slot as free."""
+    FakeSession.busy_polls = 50  # chunks take a while: the queue is planned before anyone finishes
+    reqs = _requests(9)
+    b = B.ContinuousBatcher(hbm_budget=1 << 40, max_batch=2, chunk_steps=1, planners=4)
+    try:
+        resp = b.serve(reqs)
+    finally:
+        b.close()
+    assert [r.error for r in resp if r.error] == []
+    assert len(resp) == 9
