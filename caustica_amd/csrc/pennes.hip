@@ -439,7 +439,7 @@ __global__ void k_f32_to_dose(const float* __restrict__ in, double* __restrict__
 // updated): pairs of sub-steps as one two-step sweep (k_pennes2), a remainder as one-step
 // launches (k_pennes). The launches only: nothing is allocated or synchronized.
 int launch_substeps(ThermalArgs a, float* const buf[2], int& cur, int n_substeps, int device,
-                    hipStream_t stream, Profiler* prof) {
+                    hipStream_t stream, Profiler* prof, int xch2 = 0) {
   dim3 grid((a.nz + TZ - 1) / TZ, (a.ny + TY - 1) / TY, (a.nx + XCH - 1) / XCH);
   int s = 0;
   // pairs of sub-steps: two steps per sweep over memory (k_pennes2)
@@ -448,6 +448,7 @@ int launch_substeps(ThermalArgs a, float* const buf[2], int& cur, int n_substeps
   const long yz2 = (long)((a.nz + TZ - 1) / TZ) * ((a.ny + TY2 - 1) / TY2);
   a.xch2 = 64;
   while (a.xch2 > 8 && yz2 * ((a.nx + a.xch2 - 1) / a.xch2) < 2L * ncu) a.xch2 /= 2;
+  if (xch2 > 0) a.xch2 = xch2;  // the caller's chunk length (cst_thermal_set_chunk)
   const dim3 grid2((a.nz + TZ - 1) / TZ, (a.ny + TY2 - 1) / TY2, (a.nx + a.xch2 - 1) / a.xch2);
   auto timed = [&](auto f) {
     if (prof) prof->run(0, stream, f);
@@ -631,7 +632,15 @@ int cst_thermal_run(cst_thermal* t, int n_substeps) {
   a.t_ref = t->d.t_ref;
   a.half_dt_min = t->d.half_dt_min;
   a.do_dose = t->d.dose;
-  return launch_substeps(a, t->theta, t->cur, n_substeps, t->device, t->stream, &t->prof);
+  return launch_substeps(a, t->theta, t->cur, n_substeps, t->device, t->stream, &t->prof, t->xch2);
+}
+
+int cst_thermal_set_chunk(cst_thermal* t, int planes) {
+  if (!t) return fail(CST_ERR_ARG, "null argument");
+  if (planes != 0 && planes != 8 && planes != 16 && planes != 32 && planes != 64)
+    return fail(CST_ERR_ARG, "x planes per block must be 0 (chosen per grid), 8, 16, 32 or 64");
+  t->xch2 = planes;
+  return CST_OK;
 }
 
 int cst_thermal_status(cst_thermal* t, int* nonfinite) {

@@ -20,6 +20,7 @@ struct cst_thermal {
   float* q = nullptr;  // heat source [W/m^3], float32
   unsigned* bad = nullptr;
   int cur = 0;
+  int xch2 = 0;  // x planes per block of the two-step kernel; 0: chosen per grid
   bool have_q = false;
   cst_thermal_desc d{};
   bool ready = false;

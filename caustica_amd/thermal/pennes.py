@@ -126,7 +126,7 @@ class PennesSolver:
                  arterial_temperature_c: float = ARTERIAL_TEMPERATURE_C,
                  boundary: str = "insulated", boundary_temperature_c: float | None = None,
                  reference_temperature_c: float | None = None, on_unstable: str = "refuse",
-                 finite_check_every: int = 25, profile: bool = False):
+                 finite_check_every: int = 25, profile: bool = False, x_chunk: int = 0):
         if boundary not in ("insulated", "dirichlet"):
             raise ValueError(f"boundary must be 'insulated' or 'dirichlet', got {boundary!r}")
         if on_unstable not in ("refuse", "substep"):
@@ -148,6 +148,9 @@ class PennesSolver:
         self.finite_check_every = int(finite_check_every)
         # record the stencil kernel's device time (HIP events) into result.meta
         self.profile = bool(profile)
+        # x planes per block of the two-step kernel (0: chosen per grid); results do not
+        # depend on it
+        self.x_chunk = int(x_chunk)
 
     # ------------------------------------------------------- host assembly
     def _faces(self, k, dx):
@@ -295,6 +298,8 @@ class PennesSolver:
         samples, times = [], []
         try:
             N.check(lib.cst_thermal_setup(h, C.byref(desc)))
+            if self.x_chunk:
+                N.check(lib.cst_thermal_set_chunk(h, self.x_chunk))
             if self.profile:
                 N.check(lib.cst_thermal_set_profiling(h, 1))
             if q_dev is not None:

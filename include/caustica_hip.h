@@ -248,6 +248,10 @@ int cst_thermal_destroy(cst_thermal* t);
 int cst_thermal_setup(cst_thermal* t, const cst_thermal_desc* d);
 /* Run n_substeps explicit sub-steps (asynchronous on the thermal stream). */
 int cst_thermal_run(cst_thermal* t, int n_substeps);
+/* x planes per block of the two-step stencil kernel: 0 (default) chooses per grid (the
+ * longest of 64, 32, 16, 8 that gives every CU two blocks); 8, 16, 32 or 64 force it. The
+ * results do not depend on it (each plane's update is the same arithmetic). */
+int cst_thermal_set_chunk(cst_thermal* t, int planes);
 /* 1 if any theta or dose value has stopped being finite (pennes.py:756-781). Synchronizes. */
 int cst_thermal_status(cst_thermal* t, int* nonfinite);
 /* Copy theta, theta_max (float32) and the dose (float64 sum rounded to float32); any may be NULL. */

@@ -113,3 +113,28 @@ def test_pennes_two_step_launch_equals_two_one_step_launches(boundary):
         a, b = getattr(one, name), getattr(two, name)
         print(f"[pennes] {name}: max |diff| {float(np.abs(a - b).max()):.3e}", flush=True)
         assert np.array_equal(a, b), name
+
+
+@pytest.mark.parametrize("boundary", ["insulated", "dirichlet"])
+def test_pennes_chunk_length_does_not_change_results(boundary):
+    """The two-step kernel's x-chunk length (planes per block, chosen per grid unless set)
+    only moves where blocks recompute their border planes: every length gives the same
+    bits. Ragged shape, heterogeneous coefficients, 10 steps = 5 two-step launches."""
+    from caustica_amd.thermal import PennesSolver, ThermalMedium
+
+    shape = (70, 13, 37)
+    rng = np.random.default_rng(17)
+    k = (0.2 + 0.3 * rng.random(shape)).astype(np.float32)
+    rho = (1000.0 + 80.0 * rng.random(shape)).astype(np.float32)
+    cp = (3500.0 + 300.0 * rng.random(shape)).astype(np.float32)
+    perf = (5e-4 * rng.random(shape)).astype(np.float32)
+    q = (3e7 * rng.random(shape) ** 4).astype(np.float32)
+    t0 = (42.0 + 0.5 * rng.standard_normal(shape)).astype(np.float32)
+    med = ThermalMedium(k=k, rho=rho, specific_heat=cp, perfusion=perf, dx=5e-4)
+    out = {}
+    for xch in (0, 8, 16, 32, 64):
+        s = PennesSolver("hip", boundary=boundary, finite_check_every=10, x_chunk=xch)
+        out[xch] = s.solve(t0, q, med, dt=0.05, n_steps=10, dose=True)
+    for xch, r in out.items():
+        for name in ("temperature", "temperature_max", "dose_cem43"):
+            assert np.array_equal(getattr(r, name), getattr(out[0], name)), (xch, name)
