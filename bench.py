@@ -277,12 +277,15 @@ def thermal_leg(sess, slot, plan, wl, rec_steps: int, n_heat: int, n_cool: int, 
     """Pennes + CEM43 on the 256^3 grid, heated by the resident 256^3 phasor: the heat
     source is formed on the device (alpha |P|^2 / (rho c)), scaled to 150 W/cm^2 focal
     intensity; dt 0.05 s, n_heat heating steps then n_cool cooling steps, perfused tissue
-    (k 0.5, rho C 3.6e6, w_b 0.5 kg/m^3/s), T0 = T_art = 37 C. Roofline: each launch (two
-    time steps, or one at the end of a 25-step guard interval of odd length) reads and
-    writes every array once: 48 P bytes heating and 44 P cooling (theta, Q, k,
-    1/(rho C) dt, perfusion read, theta written, theta_max and the float64 dose read and
-    written; no Q when cooling). ``one_step_schedule_GB_s`` charges those bytes per time
-    step instead, as a one-step-per-launch schedule would move them."""
+    (k 0.5, rho C 3.6e6, w_b 0.5 kg/m^3/s), T0 = T_art = 37 C. The tissue's k, perfusion
+    and dt/(rho C) maps are uniform, so the solve takes the kernels that hold those values
+    in registers; ``map_kernels`` times the same solve reading the maps (the kernels of a
+    heterogeneous medium; the same bits). Roofline: each launch (two time steps, or one at
+    the end of a 25-step guard interval of odd length) reads and writes every array it
+    uses once: theta read and written, Q read (heating only), theta_max and the float64
+    dose read and written = 36 P bytes heating and 32 P cooling, plus k, perfusion and
+    dt/(rho C) read (12 P) by the map kernels. ``one_step_schedule_GB_s`` charges those
+    bytes per time step instead, as a one-step-per-launch schedule would move them."""
     import numpy as np
 
     from caustica_amd import workloads
@@ -298,32 +301,49 @@ def thermal_leg(sess, slot, plan, wl, rec_steps: int, n_heat: int, n_cool: int, 
     src = DeviceHeatingSource.from_member(sess, slot, plan, wl.medium, wl.grid.dx,
                                           rec_steps=rec_steps).scaled(scale)
     med = workloads.pennes_tissue(shape, wl.grid.dx)
-    ps = PennesSolver("hip", profile=True)
     t0 = np.full(shape, 37.0, np.float32)
+
+    def run(uniform_maps: bool) -> dict:
+        ps = PennesSolver("hip", profile=True, uniform_maps=uniform_maps)
+        w0 = time.perf_counter()
+        hot = ps.solve(t0, src, med, dt=0.05, n_steps=n_heat, dose=True)
+        cool = ps.solve(hot.temperature, None, med, dt=0.05, n_steps=n_cool, dose=True,
+                        dose0=hot.dose_cem43)
+        wall = time.perf_counter() - w0
+        res = ThermalResult.chain([hot, cool])
+        kms = hot.meta["profile"]["kernel_ms"] + cool.meta["profile"]["kernel_ms"]
+        launches = hot.meta["profile"]["launches"] + cool.meta["profile"]["launches"]
+        maps = 0 if hot.meta["uniform_map_kernels"] else 12
+        b_heat, b_cool = 36 + maps, 32 + maps
+        one_step = (b_heat * hot.substeps * n_heat + b_cool * cool.substeps * n_cool) * P
+        moved = (b_heat * hot.meta["profile"]["launches"] + b_cool * cool.meta["profile"]["launches"]) * P
+        gbs = moved / (kms * 1e-3) / 1e9
+        if not (np.isfinite(res.temperature_max).all() and np.isfinite(res.dose_cem43).all()):
+            raise RuntimeError("thermal benchmark produced non-finite values")
+        return {"res": res, "substeps": hot.substeps, "wall_s": round(wall, 3),
+                "kernel_ms": round(kms, 3), "launches": launches,
+                "uniform_map_kernels": bool(hot.meta["uniform_map_kernels"]),
+                "roofline": {"bound": "hbm", "kernel": "k_pennes2", "achieved": round(gbs, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(gbs / HBM_PEAK_GBS, 4),
+                             "alg_bytes_per_launch": {"heating": b_heat * P, "cooling": b_cool * P}},
+                "one_step_schedule_GB_s": round(one_step / (kms * 1e-3) / 1e9, 1)}
+
     log(f"thermal: {n_heat} heating + {n_cool} cooling steps on {shape}")
-    w0 = time.perf_counter()
-    hot = ps.solve(t0, src, med, dt=0.05, n_steps=n_heat, dose=True)
-    cool = ps.solve(hot.temperature, None, med, dt=0.05, n_steps=n_cool, dose=True,
-                    dose0=hot.dose_cem43)
-    wall = time.perf_counter() - w0
-    res = ThermalResult.chain([hot, cool])
-    kms = hot.meta["profile"]["kernel_ms"] + cool.meta["profile"]["kernel_ms"]
-    launches = hot.meta["profile"]["launches"] + cool.meta["profile"]["launches"]
-    one_step = (48 * hot.substeps * n_heat + 44 * cool.substeps * n_cool) * P
-    moved = (48 * hot.meta["profile"]["launches"] + 44 * cool.meta["profile"]["launches"]) * P
-    gbs = moved / (kms * 1e-3) / 1e9
-    if not (np.isfinite(res.temperature_max).all() and np.isfinite(res.dose_cem43).all()):
-        raise RuntimeError("thermal benchmark produced non-finite values")
+    main = run(True)
+    log("thermal: the same solve on the map-reading kernels")
+    alt = run(False)
+    res, alt_res = main.pop("res"), alt.pop("res")
+    same = all(np.array_equal(getattr(res, n), getattr(alt_res, n))
+               for n in ("temperature", "temperature_max", "dose_cem43"))
     return {
-        "grid": list(shape), "dt_s": 0.05, "steps": n_heat + n_cool, "substeps": hot.substeps,
+        "grid": list(shape), "dt_s": 0.05, "steps": n_heat + n_cool, "substeps": main.pop("substeps"),
         "focal_intensity_w_cm2": workloads.FOCAL_INTENSITY_W_M2 / 1e4, "q_scale": scale,
         "peak_temperature_c": round(float(res.temperature_max.max()), 3),
         "peak_cem43_min": float(res.dose_cem43.max()),
-        "wall_s": round(wall, 3), "kernel_ms": round(kms, 3), "launches": launches,
-        "roofline": {"bound": "hbm", "kernel": "k_pennes2", "achieved": round(gbs, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "alg_bytes_per_launch": 48 * P},
-        "one_step_schedule_GB_s": round(one_step / (kms * 1e-3) / 1e9, 1),
+        **main,
+        "map_kernels": {**{k: v for k, v in alt.items() if k != "substeps"},
+                        "results_bitwise_equal": same},
     }
 
 

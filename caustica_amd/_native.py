@@ -67,6 +67,7 @@ EXPORTS = (
     "cst_thermal_setup",
     "cst_thermal_run",
     "cst_thermal_set_chunk",
+    "cst_thermal_set_uniform_maps",
     "cst_thermal_status",
     "cst_thermal_read",
     "cst_thermal_heat_from_member",
@@ -210,6 +211,7 @@ def _bind(lib):
         "cst_thermal_setup": ([v, C.POINTER(ThermalDesc)], i),
         "cst_thermal_run": ([v, i], i),
         "cst_thermal_set_chunk": ([v, i], i),
+        "cst_thermal_set_uniform_maps": ([v, i, C.POINTER(i)], i),
         "cst_thermal_status": ([v, C.POINTER(i)], i),
         "cst_thermal_read": ([v, p, p, p], i),
         "cst_thermal_heat_from_member": ([v, v, i, i, p, p, p, d, C.c_float], i),

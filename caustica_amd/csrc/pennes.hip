@@ -19,6 +19,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 #include "../../include/caustica_hip.h"
 #include "common.h"
@@ -57,7 +58,18 @@ struct ThermalArgs {
   double half_dt_min;
   int do_dose;
   int xch2;  // x planes per block of k_pennes2
+  // k, perfusion and dt/(rho C) of every voxel when all three maps are uniform (the
+  // UNI kernels take these and never read the maps)
+  float ku, pfu, cfu;
 };
+
+// Maps whose every element has the bits of element 0 leave *flag as it was; any other
+// map sets it to 1 (a plain store: every writer stores the same value).
+__global__ void __launch_bounds__(256) k_uniform_check(const unsigned* __restrict__ m, long n, unsigned* flag) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  const bool differs = i < n && m[i] != m[0];
+  if (__syncthreads_or(differs) && threadIdx.x == 0) flag[0] = 1u;
+}
 
 // Harmonic mean 2 a b / (a + b) of two conductances with the hardware reciprocal
 // (1 ulp) in place of the IEEE division sequence (thirteen instructions, one of them
@@ -117,7 +129,12 @@ struct PennesPre {
 
 // FULL: nx is a multiple of XCH, every block marches exactly XCH planes and the march is
 // unrolled completely (no loop header for the wait-count pass to merge at).
-template <bool HAS_Q, bool DOSE, bool FULL>
+// UNI: k, perfusion and dt/(rho C) are the scalars a.ku, a.pfu, a.cfu (all three maps
+// uniform); the arithmetic is the map kernel's, on the same values, so the results are
+// the same bits.
+// theta_max is stored only where it changes (a masked store leaves lines no lane writes
+// untouched in memory: most of a cooling volume).
+template <bool HAS_Q, bool DOSE, bool FULL, bool UNI>
 __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
   __shared__ float sth[2][TY + 2][TZ + 2];
   __shared__ float sk[2][TY + 2][TZ + 2];
@@ -160,7 +177,7 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
   }
   const int xlast = a.nx - 1;
   auto th = [&](int x, long c) { return a.th_in[(long)x * plane + c]; };
-  auto kk = [&](int x, long c) { return a.k[(long)x * plane + c]; };
+  auto kk = [&](int x, long c) { return UNI ? a.ku : a.k[(long)x * plane + c]; };
   // loads for plane x: its halo and coefficients, and the centre of plane x+1
   auto prefetch = [&](int x, PennesPre& p) {
     const int xc = min(x, xlast), xp = min(x + 1, xlast);
@@ -170,11 +187,13 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
     p.ht = th(xc, colh);
     p.hk = kk(xc, colh);
     p.q = HAS_Q ? a.q[i] : 0.f;
-    p.pf = a.perf[i];
-    p.cf = a.coeff[i];
+    p.pf = UNI ? a.pfu : a.perf[i];
+    p.cf = UNI ? a.cfu : a.coeff[i];
     p.tx = a.tmax[i];
     p.dz = DOSE ? a.dose[i] : 0.0;
   };
+  // k of the plane's y/z neighbours (from the LDS tile unless uniform)
+  auto sK = [&](int b, int yy, int zz) { return UNI ? a.ku : sk[b][yy][zz]; };
   const int xm = max(x0 - 1, 0);
   float tm = th(xm, col), km = kk(xm, col);
   float tc = th(x0, col), kc = kk(x0, col);
@@ -186,10 +205,10 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
   auto step = [&](int x, int b, const PennesPre& c, PennesPre& n) {
     prefetch(x + 1, n);
     sth[b][ty + 1][tz + 1] = tc;
-    sk[b][ty + 1][tz + 1] = kc;
+    if (!UNI) sk[b][ty + 1][tz + 1] = kc;
     if (hv) {
       sth[b][hly][hlz] = c.ht;
-      sk[b][hly][hlz] = c.hk;
+      if (!UNI) sk[b][hly][hlz] = c.hk;
     }
     __syncthreads();
     const float tp = c.tp, kp = c.kp;
@@ -197,11 +216,12 @@ __global__ void __launch_bounds__(TZ* TY) k_pennes(ThermalArgs a) {
     if (valid) {
       const long i = (long)x * plane + col;
       const float tn = pennes_new_theta(
-          a, x, y, z, tc, kc, tm, tp, kfm, kfp, sth[b][ty][tz + 1], sk[b][ty][tz + 1], sth[b][ty + 2][tz + 1],
-          sk[b][ty + 2][tz + 1], sth[b][ty + 1][tz], sk[b][ty + 1][tz], sth[b][ty + 1][tz + 2],
-          sk[b][ty + 1][tz + 2], HAS_Q ? c.q : 0.f, c.pf, c.cf, iw);
+          a, x, y, z, tc, kc, tm, tp, kfm, kfp, sth[b][ty][tz + 1], sK(b, ty, tz + 1), sth[b][ty + 2][tz + 1],
+          sK(b, ty + 2, tz + 1), sth[b][ty + 1][tz], sK(b, ty + 1, tz), sth[b][ty + 1][tz + 2],
+          sK(b, ty + 1, tz + 2), HAS_Q ? c.q : 0.f, c.pf, c.cf, iw);
       a.th_out[i] = tn;
-      a.tmax[i] = fmaxf(c.tx, tn);
+      const float mx = fmaxf(c.tx, tn);
+      if (__float_as_uint(mx) != __float_as_uint(c.tx)) a.tmax[i] = mx;
       bad |= !isfinite(tn);
       if (DOSE) {
         const float r_new = cem43_rate(tn + a.t_ref);
@@ -261,11 +281,12 @@ __device__ __forceinline__ void ring_cell(int r, int i, int& dy, int& dz) {
   }
 }
 
-// x planes per block (a.xch2): the longest of 64, 32, 16, 8 that still gives the grid at
-// least two blocks per CU (at 256^3: 64, 512 blocks in one round, the two recomputed
-// chunk-border planes 3 % of the work; smaller grids take shorter chunks rather than
-// leave CUs idle). The 64 x 8 tile keeps the recomputed ring at 29 % of the tile (55 %
-// for 64 x 4).
+// x planes per block (a.xch2): the longest of 64, 32, 16, 8 that still gives the grid as
+// many blocks per CU as a CU holds at once: two of the map kernels (84-93 VGPRs, five
+// waves per SIMD), four of the uniform-map ones (48-57 VGPRs, eight). At 256^3: 64 planes
+// (512 blocks) and 32 (1024 blocks), the two recomputed chunk-border planes 3 % and 6 %
+// of the work; smaller grids take shorter chunks rather than leave CUs idle. The 64 x 8
+// tile keeps the recomputed ring at 29 % of the tile (55 % for 64 x 4).
 
 // Loads of one plane step, issued one iteration before they are consumed: theta, k of
 // slots 0 and 1 at xa + 1, their maps at xa, theta, k of slot 2 at xa, theta_max and dose
@@ -277,10 +298,12 @@ struct Pennes2Pre {
   double dz;
 };
 
-template <bool HAS_Q, bool DOSE>
+// UNI and the theta_max store as in k_pennes.
+template <bool HAS_Q, bool DOSE, bool UNI>
 __global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
   __shared__ float sT[TY2 + 4][RW2];      // theta^n of plane xa on the tile + 2 rings
-  __shared__ float sK[2][TY2 + 4][RW2];   // k of planes xa (b) and xa - 1 (1 - b)
+  __shared__ float sKm[2][TY2 + 4][RW2];  // k of planes xa (b) and xa - 1 (1 - b)
+  auto sK = [&](int b, int yy, int zz) { return UNI ? a.ku : sKm[b][yy][zz]; };
   __shared__ float sU[2][TY2 + 2][RW1];   // theta^n+1 of planes xa (b) and xa - 1 (1 - b)
   const int tid = threadIdx.x, tz = tid % TZ, ty = tid / TZ;
   const int y0 = blockIdx.y * TY2, z0 = blockIdx.x * TZ;
@@ -310,17 +333,18 @@ __global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
   in[2] = in[2] && has2;
   const int xlast = a.nx - 1;
   auto F = [&](const float* m, int x, long c) { return m[(long)min(max(x, 0), xlast) * plane + c]; };
+  auto K = [&](int x, long c) { return UNI ? a.ku : F(a.k, x, c); };
   auto prefetch = [&](int xa, Pennes2Pre& p) {
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       p.t[c] = F(a.th_in, xa + 1, col[c]);
-      p.k[c] = F(a.k, xa + 1, col[c]);
+      p.k[c] = K(xa + 1, col[c]);
       p.q[c] = HAS_Q ? F(a.q, xa, col[c]) : 0.f;
-      p.pf[c] = F(a.perf, xa, col[c]);
-      p.cf[c] = F(a.coeff, xa, col[c]);
+      p.pf[c] = UNI ? a.pfu : F(a.perf, xa, col[c]);
+      p.cf[c] = UNI ? a.cfu : F(a.coeff, xa, col[c]);
     }
     p.t2 = F(a.th_in, xa, col[2]);
-    p.k2 = F(a.k, xa, col[2]);
+    p.k2 = K(xa, col[2]);
     p.tx = F(a.tmax, xa - 1, col[0]);
     p.dz = DOSE ? a.dose[(long)min(max(xa - 1, 0), xlast) * plane + col[0]] : 0.0;
   };
@@ -331,8 +355,8 @@ __global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
   for (int c = 0; c < 2; ++c) {
     tm[c] = F(a.th_in, xs - 1, col[c]);
     tc[c] = F(a.th_in, xs, col[c]);
-    kc[c] = F(a.k, xs, col[c]);
-    kfm[c] = hmean(F(a.k, xs - 1, col[c]), kc[c]);
+    kc[c] = K(xs, col[c]);
+    kfm[c] = hmean(K(xs - 1, col[c]), kc[c]);
   }
   Pennes2Pre P0, P1;
   prefetch(xs, P0);
@@ -346,14 +370,14 @@ __global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
     prefetch(xa + 1, nxt);
     // theta^n and k of plane xa on the tile and both rings
     sT[cy[0] + 2][cz[0] + 2] = tc[0];
-    sK[b][cy[0] + 2][cz[0] + 2] = kc[0];
+    if (!UNI) sKm[b][cy[0] + 2][cz[0] + 2] = kc[0];
     if (has1) {
       sT[cy[1] + 2][cz[1] + 2] = tc[1];
-      sK[b][cy[1] + 2][cz[1] + 2] = kc[1];
+      if (!UNI) sKm[b][cy[1] + 2][cz[1] + 2] = kc[1];
     }
     if (has2) {
       sT[cy[2] + 2][cz[2] + 2] = cur.t2;
-      sK[b][cy[2] + 2][cz[2] + 2] = cur.k2;
+      if (!UNI) sKm[b][cy[2] + 2][cz[2] + 2] = cur.k2;
     }
     __syncthreads();
     // step n+1 at plane xa for slots 0 and 1
@@ -368,8 +392,8 @@ __global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
         const int Y = cy[c] + 2, Z = cz[c] + 2;
         const float un = pennes_new_theta(
             a, xa, y0 + cy[c], z0 + cz[c], tc[c], kc[c], tm[c], cur.t[c], kfm[c], kfp, sT[Y - 1][Z],
-            sK[b][Y - 1][Z], sT[Y + 1][Z], sK[b][Y + 1][Z], sT[Y][Z - 1], sK[b][Y][Z - 1], sT[Y][Z + 1],
-            sK[b][Y][Z + 1], cur.q[c], cur.pf[c], cur.cf[c], iw);
+            sK(b, Y - 1, Z), sT[Y + 1][Z], sK(b, Y + 1, Z), sT[Y][Z - 1], sK(b, Y, Z - 1), sT[Y][Z + 1],
+            sK(b, Y, Z + 1), cur.q[c], cur.pf[c], cur.cf[c], iw);
         sU[b][cy[c] + 1][cz[c] + 1] = un;
         if (c == 0) u_new = un;
       }
@@ -384,11 +408,12 @@ __global__ void __launch_bounds__(TZ* TY2) k_pennes2(ThermalArgs a) {
       const int Y = cy[0] + 1, Z = cz[0] + 1, YK = cy[0] + 2, ZK = cz[0] + 2;
       const float t2n = pennes_new_theta(
           a, xb, y0 + cy[0], z0 + cz[0], u_c, kc_b, u_m, u_new, fm_b, fp_b, sU[1 - b][Y - 1][Z],
-          sK[1 - b][YK - 1][ZK], sU[1 - b][Y + 1][Z], sK[1 - b][YK + 1][ZK], sU[1 - b][Y][Z - 1],
-          sK[1 - b][YK][ZK - 1], sU[1 - b][Y][Z + 1], sK[1 - b][YK][ZK + 1], q_b, pf_b, cf_b, iw);
+          sK(1 - b, YK - 1, ZK), sU[1 - b][Y + 1][Z], sK(1 - b, YK + 1, ZK), sU[1 - b][Y][Z - 1],
+          sK(1 - b, YK, ZK - 1), sU[1 - b][Y][Z + 1], sK(1 - b, YK, ZK + 1), q_b, pf_b, cf_b, iw);
       const long i = (long)xb * plane + col[0];
       a.th_out[i] = t2n;
-      a.tmax[i] = fmaxf(fmaxf(cur.tx, u_c), t2n);
+      const float mx = fmaxf(fmaxf(cur.tx, u_c), t2n);
+      if (__float_as_uint(mx) != __float_as_uint(cur.tx)) a.tmax[i] = mx;
       bad |= !isfinite(u_c) || !isfinite(t2n);
       if (DOSE) {
         const float r0 = cem43_rate(tn_b + a.t_ref);
@@ -435,11 +460,23 @@ __global__ void k_f32_to_dose(const float* __restrict__ in, double* __restrict__
   if (i < n) d[i] = in ? (double)in[i] : 0.0;
 }
 
+// f(std::integral_constant<bool, a>, ..., <bool, c>): three run-time flags as template
+// arguments
+template <typename F>
+void with_flags(bool a, bool b, bool c, F f) {
+  auto pick = [](bool v, auto g) {
+    if (v) g(std::true_type{});
+    else g(std::false_type{});
+  };
+  pick(a, [&](auto A) { pick(b, [&](auto B) { pick(c, [&](auto C) { f(A, B, C); }); }); });
+}
+
 // n explicit sub-steps ping-ponging theta between buf[cur] and buf[1 - cur] (cur is
 // updated): pairs of sub-steps as one two-step sweep (k_pennes2), a remainder as one-step
-// launches (k_pennes). The launches only: nothing is allocated or synchronized.
+// launches (k_pennes). uni: the UNI kernels (a.ku, a.pfu, a.cfu hold the uniform maps'
+// value). The launches only: nothing is allocated or synchronized.
 int launch_substeps(ThermalArgs a, float* const buf[2], int& cur, int n_substeps, int device,
-                    hipStream_t stream, Profiler* prof, int xch2 = 0) {
+                    hipStream_t stream, Profiler* prof, int xch2 = 0, bool uni = false) {
   dim3 grid((a.nz + TZ - 1) / TZ, (a.ny + TY - 1) / TY, (a.nx + XCH - 1) / XCH);
   int s = 0;
   // pairs of sub-steps: two steps per sweep over memory (k_pennes2)
@@ -447,24 +484,23 @@ int launch_substeps(ThermalArgs a, float* const buf[2], int& cur, int n_substeps
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
   const long yz2 = (long)((a.nz + TZ - 1) / TZ) * ((a.ny + TY2 - 1) / TY2);
   a.xch2 = 64;
-  while (a.xch2 > 8 && yz2 * ((a.nx + a.xch2 - 1) / a.xch2) < 2L * ncu) a.xch2 /= 2;
+  const long resident = (uni ? 4L : 2L) * ncu;
+  while (a.xch2 > 8 && yz2 * ((a.nx + a.xch2 - 1) / a.xch2) < resident) a.xch2 /= 2;
   if (xch2 > 0) a.xch2 = xch2;  // the caller's chunk length (cst_thermal_set_chunk)
   const dim3 grid2((a.nz + TZ - 1) / TZ, (a.ny + TY2 - 1) / TY2, (a.nx + a.xch2 - 1) / a.xch2);
   auto timed = [&](auto f) {
     if (prof) prof->run(0, stream, f);
     else f();
   };
+  const bool hq = a.q != nullptr, dd = a.do_dose != 0, full = a.nx % XCH == 0;
   for (; s + 2 <= n_substeps; s += 2) {
     a.th_in = buf[cur];
     a.th_out = buf[1 - cur];
     timed([&] {
-      const bool hq = a.q != nullptr, dd = a.do_dose != 0;
-      const dim3 bl(TZ * TY2);
-      hipStream_t st = stream;
-      if (hq && dd) hipLaunchKernelGGL((k_pennes2<true, true>), grid2, bl, 0, st, a);
-      else if (hq) hipLaunchKernelGGL((k_pennes2<true, false>), grid2, bl, 0, st, a);
-      else if (dd) hipLaunchKernelGGL((k_pennes2<false, true>), grid2, bl, 0, st, a);
-      else hipLaunchKernelGGL((k_pennes2<false, false>), grid2, bl, 0, st, a);
+      with_flags(hq, dd, uni, [&](auto HQ, auto DD, auto U) {
+        hipLaunchKernelGGL((k_pennes2<decltype(HQ)::value, decltype(DD)::value, decltype(U)::value>), grid2,
+                           dim3(TZ * TY2), 0, stream, a);
+      });
       return true;
     });
     HIPCHK(hipGetLastError());
@@ -474,20 +510,14 @@ int launch_substeps(ThermalArgs a, float* const buf[2], int& cur, int n_substeps
     a.th_in = buf[cur];
     a.th_out = buf[1 - cur];
     timed([&] {
-      const bool hq = a.q != nullptr, dd = a.do_dose != 0, full = a.nx % XCH == 0;
-      const dim3 bl(TZ * TY);
-      hipStream_t st = stream;
-      if (full) {
-        if (hq && dd) hipLaunchKernelGGL((k_pennes<true, true, true>), grid, bl, 0, st, a);
-        else if (hq) hipLaunchKernelGGL((k_pennes<true, false, true>), grid, bl, 0, st, a);
-        else if (dd) hipLaunchKernelGGL((k_pennes<false, true, true>), grid, bl, 0, st, a);
-        else hipLaunchKernelGGL((k_pennes<false, false, true>), grid, bl, 0, st, a);
-      } else {
-        if (hq && dd) hipLaunchKernelGGL((k_pennes<true, true, false>), grid, bl, 0, st, a);
-        else if (hq) hipLaunchKernelGGL((k_pennes<true, false, false>), grid, bl, 0, st, a);
-        else if (dd) hipLaunchKernelGGL((k_pennes<false, true, false>), grid, bl, 0, st, a);
-        else hipLaunchKernelGGL((k_pennes<false, false, false>), grid, bl, 0, st, a);
-      }
+      with_flags(hq, dd, uni, [&](auto HQ, auto DD, auto U) {
+        if (full)
+          hipLaunchKernelGGL((k_pennes<decltype(HQ)::value, decltype(DD)::value, true, decltype(U)::value>), grid,
+                             dim3(TZ * TY), 0, stream, a);
+        else
+          hipLaunchKernelGGL((k_pennes<decltype(HQ)::value, decltype(DD)::value, false, decltype(U)::value>),
+                             grid, dim3(TZ * TY), 0, stream, a);
+      });
       return true;
     });
     HIPCHK(hipGetLastError());
@@ -602,8 +632,20 @@ int cst_thermal_setup(cst_thermal* t, const cst_thermal_desc* d) {
   hipLaunchKernelGGL(k_f32_to_dose, dim3(blocks), dim3(thr), 0, st, d->dose0 ? stage : nullptr,
                      t->dose, P);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemsetAsync(t->bad, 0, 4, st));
+  // bad[0]: the non-finite flag; bad[1..3]: k, perfusion, dt/(rho C) not uniform
+  HIPCHK(hipMemsetAsync(t->bad, 0, 16, st));
+  const float* maps[3] = {t->k, t->perf, t->coeff};
+  for (int m = 0; m < 3; ++m)
+    hipLaunchKernelGGL(k_uniform_check, dim3(blocks), dim3(thr), 0, st, reinterpret_cast<const unsigned*>(maps[m]),
+                       P, t->bad + 1 + m);
+  HIPCHK(hipGetLastError());
+  unsigned differs[3] = {1u, 1u, 1u};
+  HIPCHK(hipMemcpyAsync(differs, t->bad + 1, sizeof differs, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  t->uniform = !differs[0] && !differs[1] && !differs[2];
+  t->ku = d->k[0];
+  t->pfu = d->perf[0];
+  t->cfu = d->coeff[0];
   t->cur = 0;
   t->d = *d;
   t->ready = true;
@@ -632,7 +674,18 @@ int cst_thermal_run(cst_thermal* t, int n_substeps) {
   a.t_ref = t->d.t_ref;
   a.half_dt_min = t->d.half_dt_min;
   a.do_dose = t->d.dose;
-  return launch_substeps(a, t->theta, t->cur, n_substeps, t->device, t->stream, &t->prof, t->xch2);
+  const bool uni = t->uniform && t->allow_uniform;
+  a.ku = t->ku;
+  a.pfu = t->pfu;
+  a.cfu = t->cfu;
+  return launch_substeps(a, t->theta, t->cur, n_substeps, t->device, t->stream, &t->prof, t->xch2, uni);
+}
+
+int cst_thermal_set_uniform_maps(cst_thermal* t, int allow, int* uniform) {
+  if (!t) return fail(CST_ERR_ARG, "null argument");
+  t->allow_uniform = allow != 0;
+  if (uniform) *uniform = t->ready && t->uniform ? 1 : 0;
+  return CST_OK;
 }
 
 int cst_thermal_set_chunk(cst_thermal* t, int planes) {

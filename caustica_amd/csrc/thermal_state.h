@@ -21,6 +21,11 @@ struct cst_thermal {
   unsigned* bad = nullptr;
   int cur = 0;
   int xch2 = 0;  // x planes per block of the two-step kernel; 0: chosen per grid
+  // k, perfusion and dt/(rho C) maps all uniform (found at set-up), their values, and
+  // whether runs may then take the scalar kernels (cst_thermal_set_uniform_maps)
+  bool uniform = false;
+  bool allow_uniform = true;
+  float ku = 0.f, pfu = 0.f, cfu = 0.f;
   bool have_q = false;
   cst_thermal_desc d{};
   bool ready = false;

@@ -126,7 +126,8 @@ class PennesSolver:
                  arterial_temperature_c: float = ARTERIAL_TEMPERATURE_C,
                  boundary: str = "insulated", boundary_temperature_c: float | None = None,
                  reference_temperature_c: float | None = None, on_unstable: str = "refuse",
-                 finite_check_every: int = 25, profile: bool = False, x_chunk: int = 0):
+                 finite_check_every: int = 25, profile: bool = False, x_chunk: int = 0,
+                 uniform_maps: bool = True):
         if boundary not in ("insulated", "dirichlet"):
             raise ValueError(f"boundary must be 'insulated' or 'dirichlet', got {boundary!r}")
         if on_unstable not in ("refuse", "substep"):
@@ -151,6 +152,9 @@ class PennesSolver:
         # x planes per block of the two-step kernel (0: chosen per grid); results do not
         # depend on it
         self.x_chunk = int(x_chunk)
+        # uniform k, perfusion and dt/(rho C) maps run kernels that keep the three values
+        # in registers (same bits); False reads the maps regardless
+        self.uniform_maps = bool(uniform_maps)
 
     # ------------------------------------------------------- host assembly
     def _faces(self, k, dx):
@@ -300,6 +304,9 @@ class PennesSolver:
             N.check(lib.cst_thermal_setup(h, C.byref(desc)))
             if self.x_chunk:
                 N.check(lib.cst_thermal_set_chunk(h, self.x_chunk))
+            uni = C.c_int(0)
+            N.check(lib.cst_thermal_set_uniform_maps(h, int(self.uniform_maps), C.byref(uni)))
+            scalar_maps = bool(uni.value) and self.uniform_maps
             if self.profile:
                 N.check(lib.cst_thermal_set_profiling(h, 1))
             if q_dev is not None:
@@ -366,6 +373,7 @@ class PennesSolver:
                 "arterial_temperature_c": self.arterial_temperature_c,
                 "reference_temperature_c": self.reference_temperature_c,
                 "dose_accumulated": bool(dose),
+                "uniform_map_kernels": scalar_maps,
                 **({"profile": prof} if prof is not None else {}),
             },
         )

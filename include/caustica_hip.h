@@ -249,9 +249,17 @@ int cst_thermal_setup(cst_thermal* t, const cst_thermal_desc* d);
 /* Run n_substeps explicit sub-steps (asynchronous on the thermal stream). */
 int cst_thermal_run(cst_thermal* t, int n_substeps);
 /* x planes per block of the two-step stencil kernel: 0 (default) chooses per grid (the
- * longest of 64, 32, 16, 8 that gives every CU two blocks); 8, 16, 32 or 64 force it. The
- * results do not depend on it (each plane's update is the same arithmetic). */
+ * longest of 64, 32, 16, 8 that gives every CU as many blocks as it holds at once); 8, 16,
+ * 32 or 64 force it. The results do not depend on it (each plane's update is the same
+ * arithmetic). */
 int cst_thermal_set_chunk(cst_thermal* t, int planes);
+/* Set-up finds whether the k, perfusion and dt/(rho C) maps are all uniform (every
+ * element with element 0's bits); runs on such maps take stencil kernels that hold the
+ * three values in registers and read no map (the same arithmetic on the same values:
+ * the same bits). allow = 0 makes runs read the maps regardless; *uniform (optional)
+ * receives what set-up found (0 before set-up). No reference counterpart: the upstream
+ * medium always holds per-voxel volumes (thermal/properties.py:83-130). */
+int cst_thermal_set_uniform_maps(cst_thermal* t, int allow, int* uniform);
 /* 1 if any theta or dose value has stopped being finite (pennes.py:756-781). Synchronizes. */
 int cst_thermal_status(cst_thermal* t, int* nonfinite);
 /* Copy theta, theta_max (float32) and the dose (float64 sum rounded to float32); any may be NULL. */

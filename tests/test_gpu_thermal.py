@@ -116,6 +116,41 @@ def test_pennes_two_step_launch_equals_two_one_step_launches(boundary):
 
 
 @pytest.mark.parametrize("boundary", ["insulated", "dirichlet"])
+def test_pennes_uniform_map_kernels_equal_map_kernels(boundary):
+    """Uniform k, perfusion and rho C (the thermal benchmark's tissue) take the kernels that
+    hold the three values in registers; with those kernels switched off the same solve
+    reads the maps. Same arithmetic on the same values: the same bits, for a heated phase
+    of odd length (two-step launches and one one-step launch) and a cooling phase chained
+    onto it. Both also agree with the float64 oracle."""
+    from caustica_amd.thermal import PennesSolver, ThermalMedium
+
+    shape = (70, 13, 37)
+    print(f"[pennes] uniform-map kernels {shape} {boundary}", flush=True)
+    rng = np.random.default_rng(23)
+    one = np.ones(shape, np.float32)
+    k, rho, cp, perf = one * 0.5, one * 1050.0, one * 3600.0 / 1.05, one * np.float32(5e-4)
+    q = (3e7 * rng.random(shape) ** 4).astype(np.float32)
+    t0 = (37.0 + 0.5 * rng.standard_normal(shape)).astype(np.float32)
+    med = ThermalMedium(k=k, rho=rho, specific_heat=cp, perfusion=perf, dx=5e-4)
+    runs = {}
+    for uni in (True, False):
+        s = PennesSolver("hip", boundary=boundary, finite_check_every=25, uniform_maps=uni)
+        hot = s.solve(t0, q, med, dt=0.05, n_steps=9, dose=True)
+        cool = s.solve(hot.temperature, None, med, dt=0.05, n_steps=7, dose=True, dose0=hot.dose_cem43)
+        assert hot.meta["uniform_map_kernels"] is uni
+        runs[uni] = (hot, cool)
+    for a, b in zip(runs[True], runs[False]):
+        for name in ("temperature", "temperature_max", "dose_cem43"):
+            assert np.array_equal(getattr(a, name), getattr(b, name)), name
+    o = PennesOracle(k, rho, cp, perf, 5e-4, boundary=boundary, precision="fp64")
+    ref = o.solve(t0, q, 0.05, 9, dose=True)
+    hot = runs[True][0]
+    assert np.abs(hot.temperature - ref.temperature).max() < 1e-3
+    assert np.abs(hot.temperature_max - ref.temperature_max).max() < 1e-3
+    assert np.abs(hot.dose_cem43 - ref.dose).max() <= 1e-3 * np.abs(ref.dose).max()
+
+
+@pytest.mark.parametrize("boundary", ["insulated", "dirichlet"])
 def test_pennes_chunk_length_does_not_change_results(boundary):
     """The two-step kernel's x-chunk length (planes per block, chosen per grid unless set)
     only moves where blocks recompute their border planes: every length gives the same
