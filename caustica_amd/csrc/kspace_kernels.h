@@ -285,8 +285,8 @@ __device__ __forceinline__ Col y_column(const Shape& s, int col) {
   return c;
 }
 template <int TC>
-__device__ __forceinline__ Col x_column(const Shape& s, int col) {
-  const int M = s.nzh - 1, nt = (M + TC - 1) / TC, b = blockIdx.x;
+__device__ __forceinline__ Col x_column(const Shape& s, int col, int b = blockIdx.x) {
+  const int M = s.nzh - 1, nt = (M + TC - 1) / TC;
   Col c;
   if (b < s.ny * nt) {
     c.plane = b / nt;
@@ -581,19 +581,26 @@ __global__ void __launch_bounds__((Plan<L, TV>::TC * Plan<L, TV>::T))
   __shared__ float2 tile[L * TC];
   __shared__ float2 tw[L];
   __shared__ float2 kxt[L];
+  const MemberDev& m = a.mem[a.ids[blockIdx.z]];
+  // the first pass of a step advances the member's step clock and stores the step's
+  // scalars (read by k_zp, which runs after this launch has completed): one extra block
+  // per member (the first in x, dispatched at once) does only that, so that its chain of
+  // dependent loads delays no column tile; column tiles are blocks 1, 2, ...
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      const gptr<long long> ck = G(m.clock);
+      const long long n = ck[0];
+      ck[0] = n + 1;
+      G(m.stepv)[0] = step_scalars(m, n, ck[1]);
+    }
+    return;
+  }
   TwStage<L, TC * T> tws;
   tws.load(twg);
-  const MemberDev& m = a.mem[a.ids[blockIdx.z]];
-  // the first pass of a step advances the member's step clock (read by k_zp, which
-  // runs after this launch has completed); one thread of the member's first block
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const gptr<long long> ck = G(m.clock);
-    ck[0] = ck[0] + 1;
-  }
   KxStage<L, TC * T> kxs;
   kxs.load(m);
   const int col = threadIdx.x % TC, t = threadIdx.x / TC;
-  const Col cg = x_column<TC>(a.s, col);
+  const Col cg = x_column<TC>(a.s, col, (int)blockIdx.x - 1);
   const bool valid = cg.valid;
   const int y = cg.plane, zb = cg.zb;
   const long C = cg.stride;

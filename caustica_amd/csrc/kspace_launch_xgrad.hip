@@ -7,7 +7,8 @@ namespace cst {
 template <int N, int TV>
 static void go_xgrad(const StepArgs& a, const float2* tw, hipStream_t st) {
   using P = Plan<N, TV>;
-  dim3 g((unsigned)strided_blocks(a.s.ny, a.s.nzh, P::TC), 1, a.nmem);
+  // the column tiles and, last, the block that advances the step clock
+  dim3 g((unsigned)strided_blocks(a.s.ny, a.s.nzh, P::TC) + 1, 1, a.nmem);
   hipLaunchKernelGGL((k_xgrad<N, TV>), g, dim3(P::TC * P::T), 0, st, a, tw);
 }
 

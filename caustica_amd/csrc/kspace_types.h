@@ -57,9 +57,13 @@ struct alignas(256) MemberDev {
   // arguments at every step (a captured step replays as it is): clock[0] counts the
   // steps begun (the first pass of a step, k_xgrad, advances it), clock[1] is the
   // first recorded step. The step scalars are derived from it on the device
-  // (member_step): drive amplitude from the ramp table, drive and harmonic phases from
-  // the spp-entry (cos, sin) table of 2 pi k / spp built in float64 on the host.
+  // (step_scalars): drive amplitude from the ramp table, drive and harmonic phases from
+  // the spp-entry (cos, sin) table of 2 pi k / spp built in float64 on the host. The
+  // thread of k_xgrad that advances the clock stores them at stepv for the step's
+  // later passes: one load there, instead of a chain of dependent loads and 64-bit
+  // divisions at the start of every block.
   long long* clock;
+  struct StepMember* stepv;
   const float* amp_tab;  // float32(amplitude * ramp envelope(n dt)) for n < ramp_n
   const float2* trig;    // (cos, sin)(2 pi k / spp), k < spp
   float amp_full;        // float32(amplitude) once the ramp is complete
@@ -100,13 +104,10 @@ struct StepMember {
   int peak_idx;   // period slot for the convergence peak, -1: none
 };
 
-// Step scalars of the step in progress (engine.py:478-482 source drive, :535-550 record
-// window, :490-495 period peak), from the member's device clock. Called after the
-// step's k_xgrad has advanced clock[0].
-__device__ __forceinline__ StepMember member_step(const MemberDev& m) {
+// Step scalars of step n (engine.py:478-482 source drive, :535-550 record window,
+// :490-495 period peak), record_from the member's first recorded step.
+__device__ __forceinline__ StepMember step_scalars(const MemberDev& m, long long n, long long record_from) {
   StepMember sm;
-  const gptr<const long long> ck = G((const long long*)m.clock);
-  const long long n = ck[0] - 1, record_from = ck[1];
   const int spp = m.spp;
   const int ph = (int)(n % spp);
   const gptr<const float2> trig = G(m.trig);
@@ -120,6 +121,11 @@ __device__ __forceinline__ StepMember member_step(const MemberDev& m) {
   const long long pidx = n / spp;
   sm.peak_idx = (!sm.record && pidx < m.max_periods) ? (int)pidx : -1;
   return sm;
+}
+
+// Step scalars of the step in progress, as the step's k_xgrad stored them.
+__device__ __forceinline__ StepMember member_step(const MemberDev& m) {
+  return ld(G((const StepMember*)m.stepv), 0);
 }
 
 // Half-spectrum layout (nzh = nz/2 + 1 bins per (x, y)): bins k < nz/2 of (x, y) sit in

@@ -505,7 +505,10 @@ int member_add(cst_session* s, const cst_member_desc* d, const dl::View maps[4],
   const size_t o_rec = c.take(2 * d->n_harmonics * R * 8);
   const size_t o_pmax = c.take(R * 4);
   const size_t o_peaks = c.take((int64_t)maxp * kPeakWords * 4);
-  const size_t o_clock = c.take(sizeof(clock0));
+  // the clock (16 B) and, 64 B on, the step scalars k_xgrad stores: one 256-B slot
+  constexpr size_t kStepvAt = 64;
+  static_assert(kStepvAt >= sizeof(clock0) && kStepvAt + sizeof(StepMember) <= 256, "one 256-B slot");
+  const size_t o_clock = c.take(kStepvAt + sizeof(StepMember));
   const size_t o_amp = c.take((int64_t)std::max<size_t>(amp_tab.size(), 1) * 4);
   const size_t o_trig = c.take((int64_t)trig.size() * 8);
   HIPCHK(hipSetDevice(s->device));
@@ -560,6 +563,7 @@ int member_add(cst_session* s, const cst_member_desc* d, const dl::View maps[4],
   }
   md.nh = d->n_harmonics;
   md.clock = (long long*)(base + o_clock);
+  md.stepv = (StepMember*)(base + o_clock + kStepvAt);
   md.amp_tab = (const float*)(base + o_amp);
   md.trig = (const float2*)(base + o_trig);
   md.amp_full = (float)(d->amplitude * (0.5 * (1.0 - std::cos(M_PI * 1.0))));

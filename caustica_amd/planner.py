@@ -160,7 +160,8 @@ def hip_inventory(padded3, *, nonlinear: bool, n_harmonics: int, rec_elems: int,
         "source table": _al((nx * ny + 1) * 4) + _al(ns * 4) + _al(ns * 8),
         "record buffers": _al(2 * n_harmonics * rec_elems * 8) + _al(rec_elems * 4),
         "convergence peaks": _al(max(int(max_periods), 1) * 64 * 16 * 4),
-        "step clock and drive tables": _al(16) + _al(max(int(ramp_n), 1) * 4) + _al(int(spp) * 8),
+        # clock (16 B) and the step scalars 64 B after it (48 B) share one slot
+        "step clock and drive tables": _al(64 + 48) + _al(max(int(ramp_n), 1) * 4) + _al(int(spp) * 8),
     }
     return MemoryModel(b, int(sum(b.values())), tuple(padded3))
 
