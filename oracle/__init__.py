@@ -14,7 +14,10 @@ Modules and what each restates (upstream file:line):
   pennes.py   thermal/pennes.py:376-689, thermal/dose.py:74-89
   heating.py  sensors.py:193-295, sensors.py:356-435
   analytic.py analytic/oneill.py:33-67, analytic/planewave.py:18-57,
-              analytic/rayleigh.py:30-84
+              analytic/rayleigh.py:30-84; the power-law dispersion relation
+  staggered.py the opt-in staggered / power-law / split-PML scheme, which has no
+              upstream counterpart (upstream MILESTONES.md:1201 plans it); float64
+              only, pinned by closed forms (tests/test_oracle_staggered.py)
 
 Every function runs in one of two precisions:
   "fp32"  the upstream dtype and operation order (float32 state, complex64

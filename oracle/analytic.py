@@ -53,3 +53,22 @@ def rayleigh_pressure(points, areas, v_n, targets, k: float, rho: float = 1000.0
         r = np.sqrt(((fld[a:a + chunk, None, :] - src[None, :, :]) ** 2).sum(axis=-1))
         out[a:a + chunk] = (np.exp(1j * k * r) / r) @ strength
     return (-1j * rho * c * k / (2.0 * np.pi)) * out
+
+
+def powerlaw_wavenumber(f, alpha_f0, f0, y, c0=1500.0, iters=60):
+    """Complex wavenumber k = k_r + i alpha of a CW plane wave in the fractional-Laplacian
+    power-law medium (Treeby and Cox, JASA 127(5) 2010): omega^2 = c0^2 k^2 (1 + A),
+    A = tau i omega |k|^(y-2) - eta |k|^(y-1), tau = -2 a0 c0^(y-1),
+    eta = 2 a0 c0^y tan(pi y / 2), a0 = alpha_f0 / omega0^y; |k| taken as the real part,
+    solved by fixed-point iteration. To first order in a0: alpha = a0 omega^y and
+    1 / c_phase = 1 / c0 + a0 tan(pi y / 2) omega^(y - 1)."""
+    w = 2.0 * np.pi * float(f)
+    a0 = float(alpha_f0) / (2.0 * np.pi * float(f0)) ** y
+    tau = -2.0 * a0 * c0 ** (y - 1.0)
+    eta = 2.0 * a0 * c0**y * np.tan(np.pi * y / 2.0)
+    k = complex(w / c0)
+    for _ in range(iters):
+        kr = k.real
+        a = tau * 1j * w * kr ** (y - 2.0) - eta * kr ** (y - 1.0)
+        k = (w / c0) / np.sqrt(1.0 + a)
+    return k
