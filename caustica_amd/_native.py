@@ -21,6 +21,8 @@ CST_ERR_UNSUPPORTED = 3
 CST_ERR_STATE = 4
 MAX_BATCH = 16
 MAX_HARMONICS = 3
+SCHEME_REFERENCE = 0
+SCHEME_STAGGERED = 1
 
 #: Every symbol include/caustica_hip.h declares (checked by the CPU test suite).
 EXPORTS = (
@@ -33,6 +35,12 @@ EXPORTS = (
     "cst_session_stream",
     "cst_member_add",
     "cst_member_add_dl",
+    "cst_session_create_scheme",
+    "cst_session_scheme",
+    "cst_member_add_staggered_dl",
+    "cst_member_add_staggered",
+    "cst_member_get_density_into",
+    "cst_member_set_density_dl",
     "cst_member_remove",
     "cst_member_reset",
     "cst_session_trim",
@@ -119,6 +127,18 @@ class MemberDesc(C.Structure):
     ]
 
 
+class StaggeredDesc(C.Structure):
+    """cst_staggered_desc: tables and flags of a member of the opt-in staggered scheme."""
+
+    _fields_ = [
+        ("kfac", C.c_void_p),
+        ("pml", C.c_void_p),
+        ("alpha_power", C.c_double),
+        ("absorbing", C.c_int),
+        ("src_axes", C.c_int),
+    ]
+
+
 class ThermalDesc(C.Structure):
     _fields_ = [
         ("dx", C.c_double),
@@ -177,6 +197,14 @@ def _bind(lib):
         "cst_session_stream": ([v, C.POINTER(v)], i),
         "cst_member_add": ([v, C.POINTER(MemberDesc), C.POINTER(i)], i),
         "cst_member_add_dl": ([v, C.POINTER(MemberDesc), p, p, p, p, C.POINTER(i)], i),
+        "cst_session_create_scheme": ([i, i, i, i, i, i, C.POINTER(v)], i),
+        "cst_session_scheme": ([v, C.POINTER(i)], i),
+        "cst_member_add_staggered_dl": ([v, C.POINTER(MemberDesc), C.POINTER(StaggeredDesc), p,
+                                         C.POINTER(i)], i),
+        "cst_member_add_staggered": ([v, C.POINTER(MemberDesc), C.POINTER(StaggeredDesc), p,
+                                      C.POINTER(i)], i),
+        "cst_member_get_density_into": ([v, i, p, p, p], i),
+        "cst_member_set_density_dl": ([v, i, p, p, p], i),
         "cst_member_remove": ([v, i], i),
         "cst_member_reset": ([v, i], i),
         "cst_session_trim": ([v, C.POINTER(C.c_size_t)], i),

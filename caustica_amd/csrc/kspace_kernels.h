@@ -878,14 +878,16 @@ struct ZStage {
   float sx, sy;
 };
 
+// damping table `tab`: sz[nz] sx[nx] sy[ny] (the member's sponge, or a table of the
+// staggered scheme)
 template <int M, int ZV>
-__device__ __forceinline__ void z_stage_load(const float2* twg, const float2* rtg, const MemberDev& m,
+__device__ __forceinline__ void z_stage_load(const float2* twg, const float2* rtg, const float* tab,
                                              const Shape& sh, long line0, int nl, ZStage<M, ZV>& st) {
   // every load is unconditional (indices clamped into range): the wait-count pass can
   // then wait for exactly the loads a consumer needs instead of for all of them
   constexpr int TH = ZStage<M, ZV>::THREADS;
   constexpr int A = ZPlan<M, ZV>::A;
-  const gptr<const float2> szg = G(reinterpret_cast<const float2*>(m.sponge));
+  const gptr<const float2> szg = G(reinterpret_cast<const float2*>(tab));
 #pragma unroll
   for (int q = 0; q < ZStage<M, ZV>::TPT; ++q) {
     const int i = threadIdx.x + q * TH;
@@ -897,9 +899,15 @@ __device__ __forceinline__ void z_stage_load(const float2* twg, const float2* rt
   const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;  // block-uniform
   int x, y;
   line_xy<ZPlan<M, ZV>::LPB>(x0, y0, (int)threadIdx.x < nl ? (int)threadIdx.x : nl - 1, sh, x, y);
-  const gptr<const float> sxg = G(m.sponge) + 2 * M;
+  const gptr<const float> sxg = G(tab) + 2 * M;
   st.sx = sxg[x];
   st.sy = sxg[sh.nx + y];
+}
+
+template <int M, int ZV>
+__device__ __forceinline__ void z_stage_load(const float2* twg, const float2* rtg, const MemberDev& m,
+                                             const Shape& sh, long line0, int nl, ZStage<M, ZV>& st) {
+  z_stage_load<M, ZV>(twg, rtg, m.sponge, sh, line0, nl, st);
 }
 
 template <int M, int ZV>

@@ -14,9 +14,10 @@ namespace cst {
 
 bool length_supported(int L);
 
-// which: 0 -> sp (one spectrum), 1 -> sg[0..2] (three spectra, each transformed alone);
+// which: 0 -> sp (one spectrum), 1 -> sg[0..nspec) (each transformed alone);
 // tv: tile variant of the strided passes
-bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, int tv, hipStream_t st);
+bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, int tv, hipStream_t st,
+              int nspec = 3);
 // the 3-spectrum y passes of the step (k_ygrad after xgrad, k_ydiv after zu)
 bool launch_ygrad(int L, int tv, const StepArgs& a, const float2* tw, hipStream_t st);
 bool launch_ydiv(int L, int tv, const StepArgs& a, const float2* tw, hipStream_t st);
@@ -31,5 +32,18 @@ bool launch_zconv(int M, const StepArgs& a, const float2* tw, const float2* rt, 
 // nonlinear: some member of the batch has a beta map (picks the kernel that loads it)
 bool launch_zp(int M, int variant, const StepArgs& a, const float2* tw, const float2* rt, int update,
                bool nonlinear, hipStream_t st);
+
+// ---------------------------------------------------- opt-in staggered scheme
+// (staggered_kernels.h; the y passes of that scheme are launch_y's)
+bool launch_sxgrad(int L, const StepArgs& a, const float2* tw, hipStream_t st);
+bool launch_sygrad(int L, const StepArgs& a, const float2* tw, hipStream_t st);
+bool launch_szu(int M, const StepArgs& a, const float2* tw, const float2* rt, hipStream_t st);
+bool launch_sydiv(int L, const StepArgs& a, const float2* tw, hipStream_t st);
+bool launch_sxdiv(int L, const StepArgs& a, const float2* tw, hipStream_t st);
+// absorbing: some member of the batch carries the fractional-Laplacian terms
+bool launch_szrho(int M, const StepArgs& a, const float2* tw, const float2* rt, bool absorbing,
+                  hipStream_t st);
+bool launch_sxabs(int L, const StepArgs& a, const float2* tw, hipStream_t st);
+bool launch_szabs(int M, const StepArgs& a, const float2* tw, const float2* rt, hipStream_t st);
 
 }  // namespace cst

@@ -5,20 +5,22 @@
 namespace cst {
 
 template <int N, int TV>
-static void go_y(int dir, const StepArgs& a, const float2* tw, int which, hipStream_t st) {
+static void go_y(int dir, const StepArgs& a, const float2* tw, int which, int nspec, hipStream_t st) {
   using P = Plan<N, TV>;
-  dim3 g(strided_blocks(a.s.nx, a.s.nzh, P::TC), which == 0 ? 1 : 3, a.nmem);
+  dim3 g(strided_blocks(a.s.nx, a.s.nzh, P::TC), which == 0 ? 1 : nspec, a.nmem);
   if (dir < 0)
     hipLaunchKernelGGL((k_ypass<N, -1, TV>), g, dim3(P::TC * P::T), 0, st, a, tw, which);
   else
     hipLaunchKernelGGL((k_ypass<N, +1, TV>), g, dim3(P::TC * P::T), 0, st, a, tw, which);
 }
 
-bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, int tv, hipStream_t st) {
+bool launch_y(int L, int dir, const StepArgs& a, const float2* tw, int which, int tv, hipStream_t st,
+              int nspec) {
+  if (which != 0 && (nspec < 1 || nspec > 3)) return false;
   switch (L) {
 #define CST_Y(N)                                                                           \
   case N:                                                                                  \
-    with_tile_variant<N>(tv, [&](auto c) { go_y<N, decltype(c)::value>(dir, a, tw, which, st); }); \
+    with_tile_variant<N>(tv, [&](auto c) { go_y<N, decltype(c)::value>(dir, a, tw, which, nspec, st); }); \
     return true;
     CST_LENGTHS(CST_Y)
 #undef CST_Y

@@ -22,6 +22,28 @@ constexpr int kPeakSlots = 64;
 constexpr int kPeakStride = 16;
 constexpr int kPeakWords = kPeakSlots * kPeakStride;  // per period
 
+// Arrays of a member of the opt-in staggered scheme (session scheme CST_SCHEME_STAGGERED):
+// split density state rho_a and the maps and tables of its update. Null for members of
+// the reference scheme.
+struct StgDev {
+  float* rho[3];           // split density rho_a (real volumes); sum_a rho_a = rho
+  const float* dtrho[3];   // dt / rho0 at x_a + dx / 2 (velocity update of axis a)
+  const float* dtrho0;     // dt rho0
+  const float* c2;         // c^2
+  const float* c2tau;      // c^2 tau / dt (absorbing members; tau < 0)
+  const float* c2eta;      // c^2 eta (absorbing members)
+  const float* nl2dt;      // 2 m dt, m = 1 where beta > 0 (null: linear)
+  const float* c2bh;       // c^2 m (beta - 1) / rho0 (nonlinear members)
+  const float2* kfac;      // k_a exp(+i k_a dx / 2): x[nx], y[ny], z[nz/2 + 1]
+  const float* pml;        // 4 tables (sz[nz] sx[nx] sy[ny], pml_stride floats apart) of
+                           // exp(-sigma dt / 2): 0 at grid points on every axis, 1 + a at
+                           // staggered points x_a + dx / 2 along axis a, ones elsewhere
+  int pml_stride;          // nz + nx + ny rounded up to a multiple of 64
+  float e1, e2;            // exponents on |k|^2: (y - 2) / 2 and (y - 1) / 2
+  int absorbing;           // fractional-Laplacian absorption terms on
+  int src_axes;            // bit a set: the mass source adds to rho_a
+};
+
 // One member's descriptor, in a device table the kernels index by member slot. Entries
 // are whole 256-byte units, so that writing one member's entry (an admission while
 // other members' launches are in flight) never shares a cache line with another's.
@@ -71,6 +93,7 @@ struct alignas(256) MemberDev {
   int spp;
   int max_periods;
   int harm[kMaxHarmonics];
+  StgDev stg;
 };
 
 // Global-address-space view of a device pointer read from a table. Loads and stores

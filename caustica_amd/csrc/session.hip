@@ -97,6 +97,7 @@ struct GraphEntry {
   int ids[cst::kMaxBatch] = {};
   int v[4] = {0, 0, 0, 0};  // the session's pass variants at capture
   bool nonlinear = false;
+  bool absorbing = false;  // staggered sessions: the absorption passes are in the sequence
   int steps = 0;
   hipGraphExec_t exec = nullptr;
   uint64_t last_use = 0;
@@ -104,6 +105,7 @@ struct GraphEntry {
 
 struct cst_session {
   int device = 0;
+  int scheme = CST_SCHEME_REFERENCE;
   Shape s{};
   int max_members = 0;
   hipStream_t stream = nullptr;
@@ -180,7 +182,45 @@ bool batch_nonlinear(const cst_session* s, const StepArgs& a) {
   return false;
 }
 
+bool batch_absorbing(const cst_session* s, const StepArgs& a) {
+  return a.nmem > 0 && s->members[a.ids[0]].dev.stg.absorbing != 0;
+}
+
+// One step of the opt-in staggered scheme (the launch sequence of staggered_kernels.h).
+// Timing kinds as the reference scheme's: 0 the gradient x pass, 1 the derivative y
+// passes, 2 the velocity z pass, 3 the divergence and absorption x passes, 4 the plain y
+// passes, 5 the density and pressure z passes.
+int launch_step_staggered(cst_session* s, StepArgs& a) {
+  const Shape& sh = s->s;
+  const int M = sh.nz / 2;
+  hipStream_t st = s->stream;
+  const bool absorbing = batch_absorbing(s, a);
+  bool ok = true;
+  ok = ok && s->prof.run(0, st, [&] { return launch_sxgrad(sh.nx, a, s->tw_x, st); });
+  ok = ok && s->prof.run(1, st, [&] { return launch_sygrad(sh.ny, a, s->tw_y, st); });
+  ok = ok && s->prof.run(2, st, [&] { return launch_szu(M, a, s->tw_z, s->rt, st); });
+  ok = ok && s->prof.run(1, st, [&] { return launch_sydiv(sh.ny, a, s->tw_y, st); });
+  ok = ok && s->prof.run(3, st, [&] { return launch_sxdiv(sh.nx, a, s->tw_x, st); });
+  if (sh.ny > 1)
+    ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 1, 0, st, 3); });
+  ok = ok && s->prof.run(5, st, [&] { return launch_szrho(M, a, s->tw_z, s->rt, absorbing, st); });
+  if (absorbing) {
+    if (sh.ny > 1)
+      ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 1, 0, st, 2); });
+    ok = ok && s->prof.run(3, st, [&] { return launch_sxabs(sh.nx, a, s->tw_x, st); });
+    if (sh.ny > 1)
+      ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, +1, a, s->tw_y, 1, 0, st, 2); });
+    ok = ok && s->prof.run(5, st, [&] { return launch_szabs(M, a, s->tw_z, s->rt, st); });
+  }
+  if (sh.ny > 1)
+    ok = ok && s->prof.run(4, st, [&] { return launch_y(sh.ny, -1, a, s->tw_y, 0, 0, st); });
+  if (!ok) return fail(CST_ERR_UNSUPPORTED, "no kernel instantiated for this shape");
+  HIPCHK(hipGetLastError());
+  return CST_OK;
+}
+
 int launch_step(cst_session* s, StepArgs& a) {
+  if (s->scheme == CST_SCHEME_STAGGERED) return launch_step_staggered(s, a);
   const Shape& sh = s->s;
   const int M = sh.nz / 2;
   hipStream_t st = s->stream;
@@ -240,9 +280,10 @@ constexpr size_t kGraphCache = 48;
 // captured on first use; nullptr (and graphs off for the session) if capture fails.
 hipGraphExec_t graph_for(cst_session* s, StepArgs& a, int steps) {
   const bool nl = batch_nonlinear(s, a);
+  const bool ab = batch_absorbing(s, a);
   for (auto& g : s->gcache) {
     if (g.nmem != a.nmem || g.steps != steps || !std::equal(s->variant, s->variant + 4, g.v) ||
-        g.nonlinear != nl)
+        g.nonlinear != nl || g.absorbing != ab)
       continue;
     if (!std::equal(a.ids, a.ids + a.nmem, g.ids)) continue;
     g.last_use = ++s->gclock;
@@ -278,6 +319,7 @@ hipGraphExec_t graph_for(cst_session* s, StepArgs& a, int steps) {
   std::copy(a.ids, a.ids + a.nmem, g.ids);
   std::copy(s->variant, s->variant + 4, g.v);
   g.nonlinear = nl;
+  g.absorbing = ab;
   g.steps = steps;
   g.exec = exec;
   g.last_use = ++s->gclock;
@@ -329,8 +371,21 @@ int cst_supported_length(int n, int axis) {
 }
 
 int cst_session_create(int nx, int ny, int nz, int max_members, int device, cst_session** out) {
+  return cst_session_create_scheme(nx, ny, nz, max_members, device, CST_SCHEME_REFERENCE, out);
+}
+
+int cst_session_scheme(cst_session* s, int* scheme) {
+  if (!s || !scheme) return fail(CST_ERR_ARG, "null argument");
+  *scheme = s->scheme;
+  return CST_OK;
+}
+
+int cst_session_create_scheme(int nx, int ny, int nz, int max_members, int device, int scheme,
+                              cst_session** out) {
   if (!out) return fail(CST_ERR_ARG, "null output");
   *out = nullptr;
+  if (scheme != CST_SCHEME_REFERENCE && scheme != CST_SCHEME_STAGGERED)
+    return fail(CST_ERR_ARG, "scheme must be CST_SCHEME_REFERENCE (0) or CST_SCHEME_STAGGERED (1)");
   if (nx < 1 || ny < 1 || nz < 2 || max_members < 1 || max_members > 4096)
     return fail(CST_ERR_ARG, "invalid shape or member count");
   if (!cst_supported_length(nx, 0) || !cst_supported_length(ny, 1) || !cst_supported_length(nz, 2))
@@ -340,6 +395,7 @@ int cst_session_create(int nx, int ny, int nz, int max_members, int device, cst_
   HIPCHK(hipSetDevice(device));
   auto* s = new cst_session();
   s->device = device;
+  s->scheme = scheme;
   {
     const int ms = ((nz / 2 + 15) / 16) * 16;
     const long ps = (long)ny * ms + 16, so = (long)nx * ps;
@@ -426,15 +482,46 @@ int check_dl(const DLTensor* t, const char* what, DLDataType want, int64_t count
   return CST_OK;
 }
 
-// maps[0..3]: dt/rho, rho c^2 dt, absorb, 2 beta dt (data nullptr: linear)
-int member_add(cst_session* s, const cst_member_desc* d, const dl::View maps[4], int* slot_out) {
+// Staggered-scheme inputs of a member (cst_staggered_desc and its nine maps).
+struct StgIn {
+  const cst_staggered_desc* sd;
+  dl::View maps[9];
+};
+constexpr int kStgMaps = 9;
+
+int pml_stride(const Shape& sh) { return ((sh.nz + sh.nx + sh.ny + 63) / 64) * 64; }
+
+// maps[0..3]: dt/rho, rho c^2 dt, absorb, 2 beta dt (data nullptr: linear); stg: the
+// staggered scheme's inputs (nullptr for a reference-scheme member, whose maps they replace)
+int member_add(cst_session* s, const cst_member_desc* d, const dl::View maps[4], const StgIn* stg,
+               int* slot_out) {
   const Shape& sh = s->s;
+  if ((stg != nullptr) != (s->scheme == CST_SCHEME_STAGGERED))
+    return fail(CST_ERR_ARG, s->scheme == CST_SCHEME_STAGGERED
+                                 ? "a staggered session takes members through cst_member_add_staggered*"
+                                 : "cst_member_add_staggered* needs a session of scheme CST_SCHEME_STAGGERED");
   if (d->n_harmonics < 1 || d->n_harmonics > kMaxHarmonics)
     return fail(CST_ERR_ARG, "n_harmonics must be 1..3");
   if (d->spp < 2 || !(d->dt > 0) || !(d->f0 > 0) || !(d->amplitude > 0) || !(d->ramp_periods > 0))
     return fail(CST_ERR_ARG, "invalid timing (spp, dt, f0, amplitude, ramp)");
-  if (!maps[0].data || !maps[1].data || !maps[2].data || !d->ktab || !d->sponge)
+  if (!stg && (!maps[0].data || !maps[1].data || !maps[2].data || !d->ktab || !d->sponge))
     return fail(CST_ERR_ARG, "missing coefficient maps, k tables or sponge profiles");
+  bool absorbing = false, stg_nl = false;
+  if (stg) {
+    const cst_staggered_desc* sd = stg->sd;
+    absorbing = sd->absorbing != 0;
+    stg_nl = stg->maps[7].data != nullptr;
+    for (int i = 0; i < 5; ++i)
+      if (!stg->maps[i].data) return fail(CST_ERR_ARG, "staggered member: maps 0..4 are required");
+    if (absorbing && (!stg->maps[5].data || !stg->maps[6].data))
+      return fail(CST_ERR_ARG, "absorbing staggered member: maps 5 (c^2 tau / dt) and 6 (c^2 eta) are required");
+    if (stg_nl != (stg->maps[8].data != nullptr))
+      return fail(CST_ERR_ARG, "nonlinear staggered member: maps 7 (2 m dt) and 8 (c^2 m (beta - 1) / rho0) go together");
+    if (!sd->kfac || !sd->pml || !d->ktab) return fail(CST_ERR_ARG, "missing k tables or PML tables");
+    if (absorbing && !(sd->alpha_power > 0.0 && sd->alpha_power < 3.0 && std::fabs(sd->alpha_power - 1.0) > 1e-6))
+      return fail(CST_ERR_ARG, "alpha_power must lie in (0, 3) and differ from 1");
+    if (sd->src_axes < 0 || sd->src_axes > 7) return fail(CST_ERR_ARG, "src_axes is a 3-bit axis mask");
+  }
   if (d->n_src < 0 || (d->n_src > 0 && (!d->src_idx || !d->src_phase || !d->src_scale)))
     return fail(CST_ERR_ARG, "invalid source arrays");
   const int dims[3] = {sh.nx, sh.ny, sh.nz};
@@ -484,9 +571,11 @@ int member_add(cst_session* s, const cst_member_desc* d, const dl::View maps[4],
   const int64_t P = (int64_t)sh.nx * sh.ny * sh.nz;
   const int64_t H = sh.hs;  // bins per spectrum array (rows + Nyquist side plane)
   const int64_t R = (int64_t)(d->rec[1] - d->rec[0]) * (d->rec[3] - d->rec[2]) * (d->rec[5] - d->rec[4]);
-  const bool nonlinear = maps[3].data != nullptr;
-  const int nmaps = nonlinear ? 4 : 3;
+  const bool nonlinear = stg ? false : maps[3].data != nullptr;
+  const int nmaps = stg ? 5 + (absorbing ? 2 : 0) + (stg_nl ? 2 : 0) : (nonlinear ? 4 : 3);
   const int64_t nk = 2LL * (sh.nx + sh.ny + sh.nzh);
+  const int64_t nkf = (int64_t)sh.nx + sh.ny + sh.nzh;  // staggered shift factors (float2)
+  const int pst = pml_stride(sh);
   const int64_t nsp = sh.nz + sh.nx + sh.ny;
   const int maxp = std::max(d->max_periods, 1);
   const std::vector<float> amp_tab = ramp_table(d->amplitude, d->dt, d->f0, d->ramp_periods);
@@ -495,9 +584,12 @@ int member_add(cst_session* s, const cst_member_desc* d, const dl::View maps[4],
   Carver c;
   const size_t o_state = c.take(P * 4);      // p
   const size_t o_spec = c.take(7 * H * 8);   // sp, sg0..sg2, su0..su2 (velocity state)
+  const size_t o_rho = stg ? c.take(3 * P * 4) : 0;  // split density (staggered)
   const size_t o_maps = c.take(nmaps * P * 4);
   const size_t o_k = c.take(nk * 4);
-  const size_t o_sp = c.take(nsp * 4);
+  // reference: sponge profiles; staggered: 4 PML tables, then the shift factors
+  const size_t o_sp = stg ? c.take(4LL * pst * 4) : c.take(nsp * 4);
+  const size_t o_kf = stg ? c.take(nkf * 8) : 0;
   const size_t o_ptr = c.take((nl + 1) * 4);
   // at least one entry each: the z pass loads index 0 unconditionally
   const size_t o_sz = c.take((int64_t)std::max<size_t>(src_z.size(), 1) * 4);
@@ -541,11 +633,36 @@ int member_add(cst_session* s, const cst_member_desc* d, const dl::View maps[4],
   for (int a = 0; a < 3; ++a) md.sg[a] = spc + (a + 1) * H;
   for (int a = 0; a < 3; ++a) md.su[a] = spc + (a + 4) * H;
   float* maps_d = (float*)(base + o_maps);
-  md.dto_rho = maps_d;
-  md.rhoc2dt = maps_d + P;
-  md.absorb = maps_d + 2 * P;
-  md.beta2dt = nonlinear ? maps_d + 3 * P : nullptr;
+  if (stg) {
+    // the reference-scheme maps are never read on a staggered session
+    md.dto_rho = md.rhoc2dt = md.absorb = md.beta2dt = nullptr;
+    StgDev& g = md.stg;
+    for (int ax = 0; ax < 3; ++ax) {
+      g.rho[ax] = (float*)(base + o_rho) + ax * P;
+      g.dtrho[ax] = maps_d + ax * P;
+    }
+    g.dtrho0 = maps_d + 3 * P;
+    g.c2 = maps_d + 4 * P;
+    int k = 5;
+    g.c2tau = absorbing ? maps_d + (k++) * P : nullptr;
+    g.c2eta = absorbing ? maps_d + (k++) * P : nullptr;
+    g.nl2dt = stg_nl ? maps_d + (k++) * P : nullptr;
+    g.c2bh = stg_nl ? maps_d + (k++) * P : nullptr;
+    g.kfac = (const float2*)(base + o_kf);
+    g.pml = (const float*)(base + o_sp);
+    g.pml_stride = pst;
+    g.e1 = (float)((stg->sd->alpha_power - 2.0) / 2.0);
+    g.e2 = (float)((stg->sd->alpha_power - 1.0) / 2.0);
+    g.absorbing = absorbing ? 1 : 0;
+    g.src_axes = stg->sd->src_axes;
+  } else {
+    md.dto_rho = maps_d;
+    md.rhoc2dt = maps_d + P;
+    md.absorb = maps_d + 2 * P;
+    md.beta2dt = nonlinear ? maps_d + 3 * P : nullptr;
+  }
   md.ktab = (float*)(base + o_k);
+  // staggered: the grid-point PML table (read by the plain R2C pass of prime())
   md.sponge = (float*)(base + o_sp);
   md.src_ptr = (int*)(base + o_ptr);
   md.src_z = (int*)(base + o_sz);
@@ -592,12 +709,24 @@ int member_add(cst_session* s, const cst_member_desc* d, const dl::View maps[4],
   };
   hipError_t e = hipMemsetAsync(base + o_state, 0, P * 4, stm);
   if (e == hipSuccess) e = hipMemsetAsync(base + o_spec, 0, 7 * H * 8, stm);
-  // the maps come from host or device memory (DLPack views checked by the caller)
-  float* const dst_map[4] = {(float*)md.dto_rho, (float*)md.rhoc2dt, (float*)md.absorb, (float*)md.beta2dt};
-  for (int i = 0; i < nmaps && e == hipSuccess; ++i)
-    e = hipMemcpyAsync(dst_map[i], maps[i].data, P * 4, to_device(maps[i]), stm);
+  if (stg) {
+    if (e == hipSuccess) e = hipMemsetAsync(base + o_rho, 0, 3 * P * 4, stm);
+    int k = 0;
+    for (int i = 0; i < kStgMaps && e == hipSuccess; ++i) {
+      if (!stg->maps[i].data) continue;
+      e = hipMemcpyAsync(maps_d + (k++) * P, stg->maps[i].data, P * 4, to_device(stg->maps[i]), stm);
+    }
+    for (int t = 0; t < 4 && e == hipSuccess; ++t)
+      e = up((char*)(base + o_sp) + (size_t)t * pst * 4, stg->sd->pml + (size_t)t * nsp, nsp * 4);
+    if (e == hipSuccess) e = up(base + o_kf, stg->sd->kfac, nkf * 8);
+  } else {
+    // the maps come from host or device memory (DLPack views checked by the caller)
+    float* const dst_map[4] = {(float*)md.dto_rho, (float*)md.rhoc2dt, (float*)md.absorb, (float*)md.beta2dt};
+    for (int i = 0; i < nmaps && e == hipSuccess; ++i)
+      e = hipMemcpyAsync(dst_map[i], maps[i].data, P * 4, to_device(maps[i]), stm);
+    if (e == hipSuccess) e = up((void*)md.sponge, d->sponge, nsp * 4);
+  }
   if (e == hipSuccess) e = up((void*)md.ktab, d->ktab, nk * 4);
-  if (e == hipSuccess) e = up((void*)md.sponge, d->sponge, nsp * 4);
   if (e == hipSuccess) e = up((void*)md.src_ptr, ptr.data(), (nl + 1) * 4);
   if (e == hipSuccess) e = up((void*)md.src_z, src_z.data(), src_z.size() * 4);
   if (e == hipSuccess) e = up((void*)md.src_ab, src_ab.data(), src_ab.size() * 8);
@@ -636,7 +765,36 @@ int cst_member_add_dl(cst_session* s, const cst_member_desc* d, const DLTensor* 
     if (i == 3 && !beta2_dt) break;
     if (int r = check_dl(t[i], names[i], dl::kF32, P, s->device, &v[i])) return r;
   }
-  return member_add(s, d, v, slot);
+  return member_add(s, d, v, nullptr, slot);
+}
+
+int cst_member_add_staggered_dl(cst_session* s, const cst_member_desc* d, const cst_staggered_desc* sd,
+                                const DLTensor* const* maps, int* slot) {
+  if (!s || !d || !sd || !maps || !slot) return fail(CST_ERR_ARG, "null argument");
+  const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
+  StgIn in{};
+  in.sd = sd;
+  static const char* names[kStgMaps] = {"dtrho_x", "dtrho_y", "dtrho_z", "dtrho0", "c2",
+                                        "c2tau", "c2eta", "nl2dt", "c2bh"};
+  for (int i = 0; i < kStgMaps; ++i) {
+    if (!maps[i]) continue;
+    if (int r = check_dl(maps[i], names[i], dl::kF32, P, s->device, &in.maps[i])) return r;
+  }
+  dl::View none[4];
+  return member_add(s, d, none, &in, slot);
+}
+
+int cst_member_add_staggered(cst_session* s, const cst_member_desc* d, const cst_staggered_desc* sd,
+                             const float* const* maps, int* slot) {
+  if (!s || !d || !sd || !maps || !slot) return fail(CST_ERR_ARG, "null argument");
+  int64_t shp[3] = {s->s.nx, s->s.ny, s->s.nz};
+  DLTensor t[kStgMaps];
+  const DLTensor* tp[kStgMaps];
+  for (int i = 0; i < kStgMaps; ++i) {
+    t[i] = dl::host_tensor(const_cast<float*>(maps[i]), dl::kF32, shp, 3);
+    tp[i] = maps[i] ? &t[i] : nullptr;
+  }
+  return cst_member_add_staggered_dl(s, d, sd, tp, slot);
 }
 
 int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot_out) {
@@ -725,6 +883,8 @@ int cst_member_reset(cst_session* s, int slot) {
   HIPCHK(hipSetDevice(s->device));
   HIPCHK(hipMemsetAsync(m.dev.p, 0, P * 4, s->stream));
   HIPCHK(hipMemsetAsync(m.dev.sp, 0, 7 * H * 8, s->stream));
+  if (s->scheme == CST_SCHEME_STAGGERED)
+    for (int ax = 0; ax < 3; ++ax) HIPCHK(hipMemsetAsync(m.dev.stg.rho[ax], 0, P * 4, s->stream));
   for (int h = 0; h < m.nh; ++h) {
     HIPCHK(hipMemsetAsync(m.dev.acc[h], 0, m.rec_count * 8, s->stream));
     HIPCHK(hipMemsetAsync(m.dev.comp[h], 0, m.rec_count * 8, s->stream));
@@ -773,6 +933,9 @@ int cst_advance(cst_session* s, int n_slots, const int* slots, int n_steps) {
                                        ": its phasors were finalized by cst_member_result; reset it "
                                        "(cst_member_reset) or give it a new state before advancing it");
   }
+  for (int i = 1; i < n_slots; ++i)
+    if (s->members[slots[i]].dev.stg.absorbing != s->members[slots[0]].dev.stg.absorbing)
+      return fail(CST_ERR_ARG, "staggered members advanced together must all be absorbing or all not");
   HIPCHK(hipSetDevice(s->device));
   for (int i = 0; i < n_slots; ++i)
     if (!s->members[slots[i]].primed)
@@ -1073,9 +1236,46 @@ int cst_member_set_state(cst_session* s, int slot, int64_t n, const float* p, co
   return cst_member_set_state_dl(s, slot, n, &t[0], &t[1], &t[2], &t[3]);
 }
 
+int cst_member_get_density_into(cst_session* s, int slot, DLTensor* rho0, DLTensor* rho1, DLTensor* rho2) {
+  if (int r = check_slot(s, slot)) return r;
+  if (s->scheme != CST_SCHEME_STAGGERED) return fail(CST_ERR_STATE, "not a staggered session");
+  Member& m = s->members[slot];
+  const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
+  DLTensor* t[3] = {rho0, rho1, rho2};
+  const char* names[3] = {"rho0", "rho1", "rho2"};
+  dl::View v[3];
+  for (int i = 0; i < 3; ++i)
+    if (t[i])
+      if (int r = check_dl(t[i], names[i], dl::kF32, P, s->device, &v[i])) return r;
+  HIPCHK(hipSetDevice(s->device));
+  for (int i = 0; i < 3; ++i)
+    if (t[i]) HIPCHK(hipMemcpyAsync(v[i].data, m.dev.stg.rho[i], P * 4, from_device(v[i]), s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CST_OK;
+}
+
+int cst_member_set_density_dl(cst_session* s, int slot, const DLTensor* rho0, const DLTensor* rho1,
+                              const DLTensor* rho2) {
+  if (int r = check_slot(s, slot)) return r;
+  if (s->scheme != CST_SCHEME_STAGGERED) return fail(CST_ERR_STATE, "not a staggered session");
+  Member& m = s->members[slot];
+  const int64_t P = (int64_t)s->s.nx * s->s.ny * s->s.nz;
+  const DLTensor* t[3] = {rho0, rho1, rho2};
+  const char* names[3] = {"rho0", "rho1", "rho2"};
+  dl::View v[3];
+  for (int i = 0; i < 3; ++i)
+    if (int r = check_dl(t[i], names[i], dl::kF32, P, s->device, &v[i])) return r;
+  HIPCHK(hipSetDevice(s->device));
+  for (int i = 0; i < 3; ++i)
+    HIPCHK(hipMemcpyAsync(m.dev.stg.rho[i], v[i].data, P * 4, to_device(v[i]), s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return CST_OK;
+}
+
 static size_t raw_state_bytes(const cst_session* s) {
   const Shape& sh = s->s;
-  return (size_t)sh.nx * sh.ny * sh.nz * 4 + 3 * (size_t)sh.hs * 8;
+  const size_t pb = (size_t)sh.nx * sh.ny * sh.nz * 4;
+  return pb + 3 * (size_t)sh.hs * 8 + (s->scheme == CST_SCHEME_STAGGERED ? 3 * pb : 0);
 }
 
 int cst_member_state_size(cst_session* s, size_t* bytes) {
@@ -1094,6 +1294,10 @@ int cst_member_get_raw(cst_session* s, int slot, void* buf, size_t bytes, int64_
   HIPCHK(hipMemcpyAsync(dst, m.dev.p, pb, hipMemcpyDeviceToHost, s->stream));
   for (int ax = 0; ax < 3; ++ax)
     HIPCHK(hipMemcpyAsync(dst + pb + ax * hb, m.dev.su[ax], hb, hipMemcpyDeviceToHost, s->stream));
+  if (s->scheme == CST_SCHEME_STAGGERED)
+    for (int ax = 0; ax < 3; ++ax)
+      HIPCHK(hipMemcpyAsync(dst + pb + 3 * hb + ax * pb, m.dev.stg.rho[ax], pb, hipMemcpyDeviceToHost,
+                            s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   if (n) *n = m.n;
   return CST_OK;
@@ -1110,6 +1314,10 @@ int cst_member_set_raw(cst_session* s, int slot, const void* buf, size_t bytes, 
   HIPCHK(hipMemcpyAsync(m.dev.p, src, pb, hipMemcpyHostToDevice, s->stream));
   for (int ax = 0; ax < 3; ++ax)
     HIPCHK(hipMemcpyAsync(m.dev.su[ax], src + pb + ax * hb, hb, hipMemcpyHostToDevice, s->stream));
+  if (s->scheme == CST_SCHEME_STAGGERED)
+    for (int ax = 0; ax < 3; ++ax)
+      HIPCHK(hipMemcpyAsync(m.dev.stg.rho[ax], src + pb + 3 * hb + ax * pb, pb, hipMemcpyHostToDevice,
+                            s->stream));
   if (int r = restart_record(s, m, n)) return r;
   HIPCHK(hipStreamSynchronize(s->stream));
   m.n = n;

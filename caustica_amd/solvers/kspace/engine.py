@@ -12,6 +12,12 @@ libcaustica_hip; this module only decides how many steps to run and reads back o
 peak per period -- or, when nothing can stop the run early (convergence_tol == 0, no
 progress callback, no checkpoint), it queues the whole solve at once and reads the
 period peaks at the end.
+
+``scheme="staggered"`` runs the opt-in staggered scheme instead (half-cell shifted
+spectral derivatives, fractional-Laplacian power-law absorption alpha0 omega^y with
+``alpha_power`` y, split-field PML of strength ``pml_alpha``; see
+``caustica_amd.solvers.kspace.staggered``), which has no upstream counterpart; the
+schedule, record and result are the same.
 """
 
 from __future__ import annotations
@@ -41,6 +47,8 @@ from caustica_amd.solvers.base import (
 )
 from caustica_amd.solvers.kspace.native import KSpaceSession
 from caustica_amd.solvers.kspace.operators import MemberPlan, plan_member
+from caustica_amd.solvers.kspace.staggered import SCHEME as STAGGERED_SCHEME
+from caustica_amd.solvers.kspace.staggered import plan_staggered
 from caustica_amd.tune import autotune
 
 log = logging.getLogger("caustica")
@@ -100,9 +108,14 @@ def run_cw_kspace_pstd(
     harmonics=(1,),
     checkpoint: CheckpointSpec | None = None,
     progress: Callable[[dict], None] | None = None,
+    scheme: str = "reference",
+    alpha_power: float = 1.5,
+    pml_alpha: float = 2.0,
 ) -> SolverResult:
     harmonics = check_harmonics(harmonics)
     check_reference_point(grid, reference_point)
+    if scheme not in ("reference", "staggered"):
+        raise ValueError(f"scheme must be 'reference' or 'staggered', got {scheme!r}")
     b = get_backend(backend)
     if not b.native:
         raise SolverCapabilityError(
@@ -110,9 +123,14 @@ def run_cw_kspace_pstd(
         )
     if len(harmonics) > 3:
         raise SolverCapabilityError("the hip engine records at most 3 harmonics per solve")
-    plan = plan_member(grid, medium, source, spec, harmonics=harmonics,
-                       record_region=record_region, reference_point=reference_point,
-                       nonlinear=nonlinear)
+    if scheme == "staggered":
+        plan = plan_staggered(grid, medium, source, spec, harmonics=harmonics,
+                              record_region=record_region, reference_point=reference_point,
+                              nonlinear=nonlinear, alpha_power=alpha_power, pml_alpha=pml_alpha)
+    else:
+        plan = plan_member(grid, medium, source, spec, harmonics=harmonics,
+                           record_region=record_region, reference_point=reference_point,
+                           nonlinear=nonlinear)
     if grid.pml_vox == 0:
         log.warning(
             "grid has no PML: the FFT domain is PERIODIC and waves wrap around, producing "
@@ -125,8 +143,8 @@ def run_cw_kspace_pstd(
         extra = max(0, ceil(spec.t_end_min_us * 1e-6 / period) - spec.n_record_periods - plan.eff_max)
     max_periods = plan.eff_max + extra + spec.n_record_periods + 4
     nvox = plan.padded3[0] * plan.padded3[1] * plan.padded3[2]
-    with KSpaceSession(plan.padded3, 2, b.device) as sess:
-        if nvox >= TUNE_MIN_VOXELS:
+    with KSpaceSession(plan.padded3, 2, b.device, scheme=plan.scheme) as sess:
+        if nvox >= TUNE_MIN_VOXELS and plan.scheme == "reference":
             autotune(sess, plan)
         slot = sess.add(plan, max_periods=max_periods)
         return _control_loop(sess, slot, plan, solver_name, grid, medium, source, spec, b,
@@ -135,8 +153,8 @@ def run_cw_kspace_pstd(
 
 def _fingerprint(plan: MemberPlan, solver_name, grid, medium, source, spec, backend_name,
                  reference_point) -> dict:
-    return {
-        "scheme": SCHEME,
+    out = {
+        "scheme": SCHEME if plan.scheme == "reference" else STAGGERED_SCHEME,
         "solver": solver_name,
         "backend": backend_name,
         "grid_shape": list(grid.shape),
@@ -148,7 +166,7 @@ def _fingerprint(plan: MemberPlan, solver_name, grid, medium, source, spec, back
         "amplitude": float(source.amplitude),
         "ramp_periods": source.ramp_periods,
         "harmonics": list(plan.harmonics),
-        "nonlinear": plan.beta2_dt is not None,
+        "nonlinear": plan.beta2_dt is not None or bool(plan.stg and plan.stg["nonlinear"]),
         "record_region": [[s.start, s.stop] for s in plan.rec],
         "reference_point": None if reference_point is None else [int(r) for r in reference_point],
         "spec": spec.model_dump(),
@@ -158,6 +176,10 @@ def _fingerprint(plan: MemberPlan, solver_name, grid, medium, source, spec, back
         # the library version that wrote it
         "device_state_version": int(N.lib().cst_version()) if backend_name == "hip" else None,
     }
+    if plan.scheme == "staggered":
+        out["alpha_power"] = plan.stg["alpha_power"]
+        out["pml_alpha"] = plan.stg["pml_alpha"]
+    return out
 
 
 def _control_loop(sess: KSpaceSession, slot: int, plan: MemberPlan, solver_name, grid, medium,
@@ -186,6 +208,9 @@ def _control_loop(sess: KSpaceSession, slot: int, plan: MemberPlan, solver_name,
             f = state["fields"]
             if "device_state" in f:
                 sess.set_raw(slot, state["n"], f["device_state"])
+            elif plan.scheme == "staggered":
+                raise ValueError(f"checkpoint {checkpoint.path} holds no device state: a staggered "
+                                 f"solve resumes only from the library's own state (p, u, rho_a)")
             else:
                 u = [np.zeros(plan.padded3, np.float32) for _ in range(3)]
                 for i, c in enumerate(ucomp):
@@ -367,7 +392,8 @@ def _control_loop(sess: KSpaceSession, slot: int, plan: MemberPlan, solver_name,
             "backend": b.name,
             "padded_shape": plan.padded,
             "c_ref": plan.c_ref,
-            "nonlinear_active": plan.beta2_dt is not None,
+            "nonlinear_active": plan.beta2_dt is not None or bool(plan.stg and plan.stg["nonlinear"]),
             "source": source.label,
+            "scheme": SCHEME if plan.scheme == "reference" else STAGGERED_SCHEME,
         },
     )

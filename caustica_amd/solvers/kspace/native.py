@@ -12,6 +12,7 @@ call; ``result_export`` hands out new tensors on the host or the device.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 
 import numpy as np
@@ -24,12 +25,20 @@ KERNEL_KINDS = ("xgrad", "y3", "zu", "xdiv", "y1", "zp", "finalize")
 
 
 class KSpaceSession:
-    def __init__(self, padded3, max_members: int = 1, device: int = 0):
+    """``scheme``: "reference" (the upstream collocated scheme) or "staggered" (the opt-in
+    staggered / power-law / split-PML scheme); a session holds members of one scheme."""
+
+    def __init__(self, padded3, max_members: int = 1, device: int = 0, scheme: str = "reference"):
         self.padded3 = tuple(int(v) for v in padded3)
         self.device = int(device)
+        if scheme not in ("reference", "staggered"):
+            raise ValueError(f"scheme must be 'reference' or 'staggered', got {scheme!r}")
+        self.scheme = scheme
         self._lib = N.lib()
         h = C.c_void_p()
-        N.check(self._lib.cst_session_create(*self.padded3, int(max_members), int(device), C.byref(h)))
+        code = N.SCHEME_STAGGERED if scheme == "staggered" else N.SCHEME_REFERENCE
+        N.check(self._lib.cst_session_create_scheme(*self.padded3, int(max_members), int(device), code,
+                                                   C.byref(h)))
         self._h = h
         self.max_members = int(max_members)
         self._plans: dict[int, MemberPlan] = {}
@@ -68,6 +77,8 @@ class KSpaceSession:
         memory, copied device to device when they are on the GPU."""
         if plan.padded3 != self.padded3:
             raise ValueError(f"plan shape {plan.padded3} != session shape {self.padded3}")
+        if plan.scheme != self.scheme:
+            raise ValueError(f"a {plan.scheme} plan in a {self.scheme} session")
         d = N.MemberDesc()
         d.dx = plan.dx
         d.pml_vox = plan.pml_vox
@@ -87,12 +98,13 @@ class KSpaceSession:
             keep.append(b)
             return N.ptr(b)
 
-        if maps is None:
-            maps = [np.ascontiguousarray(a, np.float32) if a is not None else None
-                    for a in (plan.dt_over_rho, plan.rhoc2_dt, plan.absorb, plan.beta2_dt)]
-        maps = list(maps) + [None] * (4 - len(maps))
         d.ktab = arr(plan.ktab)
-        d.sponge = arr(plan.sponge)
+        if plan.scheme == "reference":
+            if maps is None:
+                maps = [np.ascontiguousarray(a, np.float32) if a is not None else None
+                        for a in (plan.dt_over_rho, plan.rhoc2_dt, plan.absorb, plan.beta2_dt)]
+            maps = list(maps) + [None] * (4 - len(maps))
+            d.sponge = arr(plan.sponge)
         d.n_src = int(plan.src_idx3.shape[0])
         d.src_idx = arr(plan.src_idx3, np.int32)
         d.src_phase = arr(plan.src_phase)
@@ -109,9 +121,24 @@ class KSpaceSession:
             max_periods = plan.eff_max + 4
         d.max_periods = int(max_periods)
         slot = C.c_int(-1)
-        with D.borrow(maps[0]) as m0, D.borrow(maps[1]) as m1, D.borrow(maps[2]) as m2, \
-                D.borrow(maps[3]) as m3:
-            N.check(self._lib.cst_member_add_dl(self._h, C.byref(d), m0, m1, m2, m3, C.byref(slot)))
+        if plan.scheme == "staggered":
+            g = plan.stg
+            sd = N.StaggeredDesc()
+            sd.kfac = arr(g["kfac"], np.complex64)
+            sd.pml = arr(g["pml"])
+            sd.alpha_power = float(g["alpha_power"])
+            sd.absorbing = 1 if g["absorbing"] else 0
+            sd.src_axes = int(g["src_axes"])
+            smaps = list(maps) if maps is not None else list(g["maps"])
+            with contextlib.ExitStack() as ex:
+                ts = [ex.enter_context(D.borrow(a)) for a in smaps]
+                tarr = (C.c_void_p * 9)(*[C.cast(t, C.c_void_p) if t is not None else None for t in ts])
+                N.check(self._lib.cst_member_add_staggered_dl(self._h, C.byref(d), C.byref(sd), tarr,
+                                                              C.byref(slot)))
+        else:
+            with D.borrow(maps[0]) as m0, D.borrow(maps[1]) as m1, D.borrow(maps[2]) as m2, \
+                    D.borrow(maps[3]) as m3:
+                N.check(self._lib.cst_member_add_dl(self._h, C.byref(d), m0, m1, m2, m3, C.byref(slot)))
         self._plans[slot.value] = plan
         return slot.value
 
@@ -231,6 +258,20 @@ class KSpaceSession:
         with D.borrow(p) as a0, D.borrow(u0) as a1, D.borrow(u1) as a2, D.borrow(u2) as a3:
             N.check(self._lib.cst_member_get_state_into(self._h, int(slot), a0, a1, a2, a3, C.byref(n)))
         return int(n.value)
+
+    def get_density(self, slot: int):
+        """[rho_0, rho_1, rho_2] of a staggered member (real volumes of the padded shape)."""
+        out = [np.empty(self.padded3, np.float32) for _ in range(3)]
+        with D.borrow(out[0]) as a0, D.borrow(out[1]) as a1, D.borrow(out[2]) as a2:
+            N.check(self._lib.cst_member_get_density_into(self._h, int(slot), a0, a1, a2))
+        return out
+
+    def set_density(self, slot: int, rho0, rho1, rho2) -> None:
+        """Restore a staggered member's split density (after ``set_state``)."""
+        arrs = [np.ascontiguousarray(np.reshape(a, self.padded3), np.float32) if isinstance(a, np.ndarray)
+                else a for a in (rho0, rho1, rho2)]
+        with D.borrow(arrs[0]) as a0, D.borrow(arrs[1]) as a1, D.borrow(arrs[2]) as a2:
+            N.check(self._lib.cst_member_set_density_dl(self._h, int(slot), a0, a1, a2))
 
     def set_variant(self, pass_: int, variant: int) -> None:
         N.check(self._lib.cst_session_set_variant(self._h, int(pass_), int(variant)))

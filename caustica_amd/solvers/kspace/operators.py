@@ -166,6 +166,10 @@ class MemberPlan:
     eff_max: int
     label: str = ""
     meta: dict = field(default_factory=dict)
+    #: "reference" (the upstream scheme) or "staggered" (the opt-in scheme, whose maps and
+    #: tables are in ``stg``; see caustica_amd.solvers.kspace.staggered)
+    scheme: str = "reference"
+    stg: dict | None = None
 
     @property
     def rec_shape(self) -> tuple:
@@ -178,6 +182,10 @@ class MemberPlan:
         # one spectrum: rows of nz/2 bins padded to 16, planes + 16, Nyquist side plane
         H = nx * (ny * ((nz // 2 + 15) // 16 * 16) + 16) + (nx * ny + 15) // 16 * 16
         R = int(np.prod(self.rec_shape))
+        if self.scheme == "staggered":
+            # p, split density rho_a, and the maps given (dt/rho0' x3, dt rho0, c^2, ...)
+            nmaps = sum(m is not None for m in self.stg["maps"])
+            return int(16 * P + 56 * H + 4 * nmaps * P + R * (16 * len(self.harmonics) + 4) + (1 << 20))
         nmaps = 3 + (self.beta2_dt is not None)
         return int(4 * P + 56 * H + 4 * nmaps * P + R * (16 * len(self.harmonics) + 4) + (1 << 20))
 

@@ -23,11 +23,16 @@ class WesterveltKSpacePSTD(SolverBase):
         nonlinear=True,
         drive=frozenset({"cw"}),
         backends=frozenset({"hip"}),
+        absorption=frozenset({"exponential", "power_law"}),
     )
 
     def run(self, grid, medium, source, spec: CWRunSpec | None = None, backend: str = "auto",
             record_region=None, reference_point=None, harmonics=(1,), checkpoint=None,
-            progress=None, **kwargs: Any) -> SolverResult:
+            progress=None, *, scheme: str = "reference", alpha_power: float = 1.5,
+            pml_alpha: float = 2.0, **kwargs: Any) -> SolverResult:
+        """``scheme="staggered"`` selects the opt-in staggered / power-law / split-PML
+        scheme (no upstream counterpart; ``alpha_power`` y of alpha0 omega^y, ``pml_alpha``
+        the PML strength); the default runs the upstream scheme."""
         if kwargs:
             raise TypeError(f"unknown run() options: {sorted(kwargs)}")
         spec = spec or CWRunSpec()
@@ -35,4 +40,5 @@ class WesterveltKSpacePSTD(SolverBase):
         return run_cw_kspace_pstd(self.name, grid, medium, source, spec, backend=backend,
                                   record_region=record_region, reference_point=reference_point,
                                   nonlinear=True, harmonics=harmonics, checkpoint=checkpoint,
-                                  progress=progress)
+                                  progress=progress, scheme=scheme, alpha_power=alpha_power,
+                                  pml_alpha=pml_alpha)

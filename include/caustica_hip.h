@@ -109,6 +109,60 @@ int cst_member_add(cst_session* s, const cst_member_desc* d, int* slot);
 int cst_member_add_dl(cst_session* s, const cst_member_desc* d, const DLTensor* dt_over_rho,
                       const DLTensor* rhoc2_dt, const DLTensor* absorb, const DLTensor* beta2_dt,
                       int* slot);
+/* ------------------------------------------------ opt-in staggered scheme
+ * No counterpart in the reference: its step (engine.py:458-482) is collocated with
+ * exp(-alpha c dt) absorption and a multiplicative sponge, and power-law absorption is
+ * its unstarted milestone M16 (MILESTONES.md:1201). A session created with
+ * CST_SCHEME_STAGGERED steps the first-order equations on a staggered grid: half-cell
+ * shifted, kappa-corrected spectral derivatives, split density rho_a with a split-field
+ * PML, fractional-Laplacian power-law absorption and dispersion (alpha0 omega^y, when
+ * `absorbing`) and the Westervelt term m (beta - 1) rho^2 / rho0. The schedule, source
+ * drive, period peaks, record window, results and state calls are the reference
+ * scheme's; the source adds q(n) / (nd c^2) to rho_a for the grid's axes (src_scale =
+ * float32(2 c dt / dx / (nd c^2)) at each voxel). Members of a reference-scheme session
+ * are added with cst_member_add*, of a staggered session with cst_member_add_staggered*
+ * (the other form is refused). cst_session_create = scheme CST_SCHEME_REFERENCE. */
+#define CST_SCHEME_REFERENCE 0
+#define CST_SCHEME_STAGGERED 1
+
+int cst_session_create_scheme(int nx, int ny, int nz, int max_members, int device, int scheme,
+                              cst_session** out);
+int cst_session_scheme(cst_session* s, int* scheme);
+
+typedef struct {
+  /* complex64 [nx + ny + nz/2 + 1] (interleaved re, im): f_a(k) = k_a exp(+i k_a dx / 2)
+   * per axis, k_a = 2 pi fftfreq (rfftfreq along z); i kappa f_a is the derivative to
+   * x_a + dx/2, i kappa conj(f_a) the one back */
+  const float* kfac;
+  /* float32 [4][nz + nx + ny], each table sz[nz] sx[nx] sy[ny] of exp(-sigma dt / 2):
+   * table 0 at grid points on every axis (density), table 1 + a at staggered points
+   * x_a + dx/2 along axis a and ones along the others (velocity of axis a) */
+  const float* pml;
+  double alpha_power;   /* y of alpha0 omega^y (0 < y < 3, y != 1) */
+  int absorbing;        /* 1: maps 5, 6 are given and the fractional Laplacians run */
+  int src_axes;         /* bit a: the source adds to rho_a (the grid's axes) */
+} cst_staggered_desc;
+
+/* Upload a member of a staggered session. The desc's ktab (full wavenumbers; the _d
+ * entries are unused), source, boxes, timing and harmonics mean what they mean for
+ * cst_member_add; its map and sponge pointers are ignored. maps: 9 float32 tensors of
+ * nx * ny * nz elements (host or device): [0..2] dt / rho0 at x_a + dx/2, [3] dt rho0,
+ * [4] c^2, [5] c^2 tau / dt and [6] c^2 eta (absorbing members, else NULL; tau =
+ * -2 alpha0 c^(y-1), eta = 2 alpha0 c^y tan(pi y / 2)), [7] 2 m dt and [8] c^2 m
+ * (beta - 1) / rho0 (nonlinear members, else NULL; m = 1 where beta > 0). Members
+ * advanced together must agree on `absorbing` (CST_ERR_ARG otherwise). */
+int cst_member_add_staggered_dl(cst_session* s, const cst_member_desc* d, const cst_staggered_desc* sd,
+                                const DLTensor* const* maps, int* slot);
+int cst_member_add_staggered(cst_session* s, const cst_member_desc* d, const cst_staggered_desc* sd,
+                             const float* const* maps, int* slot);
+/* The split density rho_0, rho_1, rho_2 (real volumes of the padded shape, host or device
+ * tensors; any may be NULL in get) of a staggered member. set_density after
+ * cst_member_set_state_dl (which restarts the record window and sets n) completes the
+ * state of step n. Synchronizes. */
+int cst_member_get_density_into(cst_session* s, int slot, DLTensor* rho0, DLTensor* rho1, DLTensor* rho2);
+int cst_member_set_density_dl(cst_session* s, int slot, const DLTensor* rho0, const DLTensor* rho1,
+                              const DLTensor* rho2);
+
 /* Retire a member once its own queued steps are done (steps of other members queued
  * after them may still run). Its device block stays with the session and is reused by
  * the next member of the same footprint (no hipFree on the serving path). */
@@ -191,7 +245,8 @@ int cst_member_get_state_into(cst_session* s, int slot, DLTensor* p, DLTensor* u
                               DLTensor* u2, int64_t* n);
 int cst_member_set_state_dl(cst_session* s, int slot, int64_t n, const DLTensor* p, const DLTensor* u0,
                             const DLTensor* u1, const DLTensor* u2);
-/* The exact device state as bytes (p, then the three velocity spectra), for checkpoints
+/* The exact device state as bytes (p, then the three velocity spectra, then on a
+ * staggered session the three density components), for checkpoints
  * that resume bit for bit (io/checkpoint.py:127-226 with the resume contract of
  * tests/test_checkpoint.py:61-83). state_size gives the byte count for the session's
  * shape; get_raw / set_raw refuse any other size. Synchronizes. */
