@@ -273,6 +273,102 @@ def westervelt_leg(log) -> dict:
             "step": step, "passes": passes}
 
 
+def staggered_kind_bytes(padded3, absorbing: bool, nonlinear: bool) -> dict:
+    """Algorithmic HBM bytes per time step of the opt-in staggered scheme, per timing kind
+    (every array a pass must read or write counted once; H half-spectrum bins, P voxels):
+    sxgrad reads p-hat and writes two spectra (24 H); the derivative y passes move 40 H
+    (gradient) and 48 H (divergence: three spectra in, three out); szu reads each
+    gradient and velocity spectrum and one dt/rho0' map per axis and writes the velocity
+    spectra (72 H + 12 P); sxdiv moves 48 H, sxabs 32 H; the plain y passes 48 H (3
+    spectra), and with absorption 32 H twice more, plus 16 H for p; szrho reads the three
+    divergence spectra, rho_a and dt rho0 (+ 2 m dt) and writes rho_a -- absorption-free
+    it also reads c^2 (+ the Westervelt map) and writes p and its spectrum (32 H + 36 P),
+    absorbing it writes rho and two spectra (40 H + 32 P) and szabs reads two spectra, rho,
+    c^2, c^2 tau / dt, c^2 eta (+ the Westervelt map) and writes p and its spectrum
+    (24 H + 20 P)."""
+    nx, ny, nz = padded3
+    P = nx * ny * nz
+    H = nx * ny * (nz // 2 + 1)
+    nl = 4 * P if nonlinear else 0
+    if absorbing:
+        zp = (40 * H + 32 * P + nl) + (24 * H + 20 * P + nl)
+        x = 48 * H + 32 * H
+        y = 48 * H + 32 * H + 32 * H + 16 * H
+    else:
+        zp = 32 * H + 36 * P + 2 * nl
+        x = 48 * H
+        y = 48 * H + 16 * H
+    return {"xgrad": 24 * H, "y3": 88 * H, "zu": 72 * H + 12 * P, "xdiv": x, "y1": y, "zp": zp}
+
+
+def staggered_leg(log) -> dict:
+    """The opt-in staggered scheme (no upstream counterpart) on the 256^3 heterogeneous
+    medium of the headline workload: power-law absorption (y = 1.5, alpha0 from each
+    tissue's absorption at 1 MHz), split-field PML on the 16-cell layer, the same bowl,
+    CFL and 900-step schedule. One untimed and one timed solve with the member resident,
+    then a profiled one: per timing kind and per step, the algorithmic bytes of
+    staggered_kind_bytes over the HIP-event durations."""
+    import numpy as np
+
+    from caustica_amd import _native as N
+    from caustica_amd import workloads
+    from caustica_amd.solvers.kspace.native import KSpaceSession
+    from caustica_amd.solvers.kspace.staggered import plan_staggered
+
+    log("staggered: building the 256^3 workload")
+    wl = workloads.cfg1_hetero_linear()
+    plan = plan_staggered(wl.grid, wl.medium, wl.source, wl.spec, harmonics=wl.harmonics,
+                          reference_point=wl.reference_point, alpha_power=1.5)
+    spp = plan.spp
+    rec = wl.spec.n_record_periods * spp
+    total = wl.periods_total * spp
+    absorbing, nonlinear = plan.stg["absorbing"], plan.stg["nonlinear"]
+    nb = staggered_kind_bytes(plan.padded3, absorbing, nonlinear)
+    with KSpaceSession(plan.padded3, 1, scheme="staggered") as sess:
+        slot = sess.add(plan, max_periods=wl.periods_total + 4)
+
+        def one_solve():
+            sess.reset(slot)
+            sess.schedule(slot, total - rec)
+            sess.advance([slot], total)
+            N.check(N.lib().cst_member_result(sess.handle, slot, 0, 2.0 / rec, None, None))
+
+        one_solve()
+        sess.sync()
+        t0 = time.perf_counter()
+        one_solve()
+        sess.sync()
+        dt = time.perf_counter() - t0
+        peaks = sess.peaks(slot, 0, wl.periods_total - wl.spec.n_record_periods)
+        if not np.isfinite(peaks).all():
+            raise RuntimeError("staggered benchmark produced non-finite values")
+        sess.set_profiling(True)
+        one_solve()
+        sess.sync()
+        kt = sess.kernel_times()
+        sess.set_profiling(False)
+    kinds = {}
+    step_ms = 0.0
+    for name, (ms, cnt) in kt.items():
+        if cnt == 0 or name not in nb:
+            continue
+        per_step = ms / total
+        step_ms += per_step
+        gbs = nb[name] / (per_step * 1e-3) / 1e9
+        kinds[name] = {"launches": cnt, "us_per_step": round(per_step * 1e3, 2),
+                       "bytes_per_step": nb[name], "GB_s": round(gbs, 1),
+                       "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    step_bytes = sum(nb.values())
+    gbs = step_bytes / (step_ms * 1e-3) / 1e9
+    return {"workload": f"staggered_powerlaw_{wl.name}", "grid": list(wl.grid.shape), "spp": spp,
+            "steps_per_solve": total, "alpha_power": 1.5, "absorbing": absorbing,
+            "solves_per_s": round(1.0 / dt, 4), "ms_per_solve": round(dt * 1e3, 1),
+            "ms_per_time_step": round(dt * 1e3 / total, 4),
+            "step": {"alg_bytes": step_bytes, "kernel_ms": round(step_ms, 4), "GB_s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)},
+            "kinds": kinds}
+
+
 def thermal_leg(sess, slot, plan, wl, rec_steps: int, n_heat: int, n_cool: int, log) -> dict:
     """Pennes + CEM43 on the 256^3 grid, heated by the resident 256^3 phasor: the heat
     source is formed on the device (alpha |P|^2 / (rho c)), scaled to 150 W/cm^2 focal
@@ -489,8 +585,13 @@ def main() -> int:
     ap.add_argument("--cpu-legs", default="headline", choices=["headline", "all"],
                     help="CPU baseline of the headline workload only, or of every configuration")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every pass instead of replaying captured HIP graphs (the same "
+                         "kernels and bits; for tools that cannot trace graph launches)")
     ap.add_argument("--no-westervelt", dest="westervelt", action="store_false",
                     help="skip the 256x256x384 Westervelt leg")
+    ap.add_argument("--no-staggered", dest="staggered", action="store_false",
+                    help="skip the leg of the opt-in staggered / power-law scheme (256^3)")
     ap.add_argument("--thermal-steps", type=int, default=600,
                     help="Pennes steps heated by the 256^3 phasor (1/3 heating, 2/3 cooling; 0: skip)")
     ap.add_argument("--serving-requests", type=int, default=48,
@@ -527,6 +628,8 @@ def main() -> int:
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     get_backend("hip")
+    if args.no_graphs:
+        KSpaceSession.graphs_default = False
     rates = [float(v) for v in args.serving_rates.split(",") if v]
     serial_mode = args.serving_serial
     if serial_mode == "auto":
@@ -589,7 +692,7 @@ def main() -> int:
     one_solve()
     sess.sync()
     direct_ms = (time.perf_counter() - t0) * 1e3
-    sess.set_graphs(True)
+    sess.set_graphs(KSpaceSession.graphs_default)
     graphs = {"replayed": sess.graph_stats()["enabled"],
               "ms_per_solve_graphs": round(elapsed / args.steps * 1e3, 2),
               "ms_per_solve_direct": round(direct_ms, 2)}
@@ -616,6 +719,10 @@ def main() -> int:
     if args.westervelt and args.workload == "cfg1":
         westervelt = westervelt_leg(log)
         log(f"westervelt: {json.dumps(westervelt)}")
+    staggered = None
+    if args.staggered and args.workload == "cfg1":
+        staggered = staggered_leg(log)
+        log(f"staggered: {json.dumps(staggered)}")
     thermal = None
     if args.thermal_steps > 0 and args.workload == "cfg1":
         n_heat = max(1, args.thermal_steps // 3)
@@ -667,6 +774,7 @@ def main() -> int:
         "roofline": roofline,
         "passes": passes,
         "westervelt": westervelt,
+        "staggered": staggered,
         "thermal": thermal,
         "serving": serving,
         "cpu_baseline": cpu,

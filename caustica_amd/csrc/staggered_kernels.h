@@ -552,6 +552,11 @@ __global__ void __launch_bounds__((ZPlan<M, 0>::THREADS))
   z_spec_put<M, 0>(nl, sr, srn, s);
   __syncthreads();
   z_spec_load<M, 0>(G(m.sg[1]), line0, nl, a.s, sr, srn);
+  float2 dto0[ABS ? CN::EPT : 1];  // dt rho0 (absorbing: the weight of sum_a d_a)
+  if constexpr (ABS) {
+#pragma unroll
+    for (int j = 0; j < CN::EPT; ++j) dto0[j] = fac[j];
+  }
   if (nonlin) {
     float2 nl2[CN::EPT];
     z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.nl2dt)) + e0, nl, nl2);
@@ -659,8 +664,6 @@ __global__ void __launch_bounds__((ZPlan<M, 0>::THREADS))
     __syncthreads();
     z_r2c<M, 0>(ll, t, lvalid, s, G(m.sp) + s.row[lvalid ? ll : 0], G(m.sp) + a.s.so + line0 + (lvalid ? ll : 0));
   } else {
-    float2 dto0[CN::EPT];
-    z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.dtrho0)) + e0, nl, dto0);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
       const int l = ZElem<M, 0>::line(j), mm = ZElem<M, 0>::pos(j, l);

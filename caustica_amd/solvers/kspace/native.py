@@ -26,7 +26,11 @@ KERNEL_KINDS = ("xgrad", "y3", "zu", "xdiv", "y1", "zp", "finalize")
 
 class KSpaceSession:
     """``scheme``: "reference" (the upstream collocated scheme) or "staggered" (the opt-in
-    staggered / power-law / split-PML scheme); a session holds members of one scheme."""
+    staggered / power-law / split-PML scheme); a session holds members of one scheme.
+    ``graphs_default`` (class-wide) is whether new sessions replay captured HIP graphs of
+    the step sequence or launch every pass; the results are the same bits either way."""
+
+    graphs_default = True
 
     def __init__(self, padded3, max_members: int = 1, device: int = 0, scheme: str = "reference"):
         self.padded3 = tuple(int(v) for v in padded3)
@@ -40,6 +44,8 @@ class KSpaceSession:
         N.check(self._lib.cst_session_create_scheme(*self.padded3, int(max_members), int(device), code,
                                                    C.byref(h)))
         self._h = h
+        if not KSpaceSession.graphs_default:
+            N.check(self._lib.cst_session_set_graphs(self._h, 0))
         self.max_members = int(max_members)
         self._plans: dict[int, MemberPlan] = {}
 
