@@ -64,3 +64,32 @@ def test_foreign_checkpoint_is_refused(tmp_path):
     other.source = dataclasses.replace(other.source, amplitude=other.source.amplitude * 2.0)
     with pytest.raises(CheckpointMismatch):
         _run(other, checkpoint=CheckpointSpec(ck, every_periods=1))
+
+
+def test_staggered_resume_is_bitwise_and_the_scheme_is_fingerprinted(tmp_path):
+    """The opt-in staggered scheme resumes from the library's raw state (p, the velocity
+    spectra and the split density) bit for bit, and a checkpoint of the reference scheme
+    does not resume a staggered solve (nor the other way round)."""
+    req = _request(seed=9, solver="westervelt")
+    kw = dict(scheme="staggered", alpha_power=1.1)
+    print("[checkpoint] staggered: uninterrupted", flush=True)
+    ref = _run(req, **kw)
+    calls = {"n": 0}
+
+    def stop_after_three():
+        calls["n"] += 1
+        return calls["n"] >= 3
+
+    ck = tmp_path / "stg.ckpt.npz"
+    print("[checkpoint] staggered: interrupted", flush=True)
+    with pytest.raises(RunInterrupted):
+        _run(req, checkpoint=CheckpointSpec(ck, every_periods=2, stop_when=stop_after_three), **kw)
+    print("[checkpoint] staggered: resumed", flush=True)
+    res = _run(req, checkpoint=CheckpointSpec(ck, every_periods=2, keep_on_success=True), **kw)
+    assert res.steps_total == ref.steps_total
+    for h in ref.phasors:
+        np.testing.assert_array_equal(res.phasors[h], ref.phasors[h])
+    np.testing.assert_array_equal(res.p_max, ref.p_max)
+    assert res.meta["scheme"] == "cw-kspace-pstd-staggered/1"
+    with pytest.raises(CheckpointMismatch):
+        _run(req, checkpoint=CheckpointSpec(ck, every_periods=2))  # the reference scheme
