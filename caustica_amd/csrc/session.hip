@@ -5,7 +5,7 @@
 // Streams: every step of a session runs on its compute stream, in order; member
 // uploads run on an upload stream (the host waits for them before cst_member_add
 // returns, so nothing on the compute stream has to wait); read-backs (period peaks,
-// finalized phasors) run on an io stream that first waits for the member's last
+// finalized phasors) run on an io stream once the host has waited for the member's last
 // queued step (an event recorded after each cst_advance), so that reading a finished
 // member back overlaps the next steps of the others.
 #include <hip/hip_runtime.h>
@@ -1005,7 +1005,10 @@ int cst_member_peaks(cst_session* s, int slot, int first, int count, float* out)
   if (count == 0) return CST_OK;
   HIPCHK(hipSetDevice(s->device));
   std::vector<unsigned> raw((size_t)count * kPeakWords);
-  HIPCHK(hipStreamWaitEvent(s->io, m.done, 0));
+  // a host wait for the member's last queued step, not a stream dependency: another thread
+  // may be capturing the compute stream (the next steps of the other members) meanwhile,
+  // and a dependency of the io stream on an event of a capturing stream is refused
+  HIPCHK(hipEventSynchronize(m.done));
   HIPCHK(hipMemcpyAsync(raw.data(), m.dev.peaks + (size_t)first * kPeakWords,
                         raw.size() * 4, hipMemcpyDeviceToHost, s->io));
   HIPCHK(hipStreamSynchronize(s->io));
@@ -1032,7 +1035,7 @@ int cst_member_result_into(cst_session* s, int slot, int hi, double scale, DLTen
   if (pmax)
     if (int r = check_dl(pmax, "pmax", dl::kF32, R, s->device, &vm)) return r;
   HIPCHK(hipSetDevice(s->device));
-  HIPCHK(hipStreamWaitEvent(s->io, m.done, 0));
+  HIPCHK(hipEventSynchronize(m.done));  // a host wait, as in cst_member_peaks
   if (!m.finalized[hi]) {
     const int threads = 256;
     const int64_t blocks = (R + threads - 1) / threads;

@@ -9,8 +9,10 @@ because it counts dense operator volumes the device never builds). Every session
 advances all of its members together -- one batched launch sequence per step, each
 member at its own step counter, so members join and leave at any step boundary --
 and different shapes run on their own streams, so their kernels overlap on the
-device. A member whose schedule is done is read back and retired; its slot and memory
-are reused by the next admission of that shape.
+device. A member whose schedule is done is read back and retired on a read-back
+thread (its copies run on the session's read-back stream while the serving loop keeps
+admitting and advancing the others); its slot and memory are reused by the next
+admission of that shape once it is retired.
 
 A member's trajectory does not depend on what it shares launches with: every kernel
 reads only the member's own arrays and scalars, so a result is bit-identical alone
@@ -84,6 +86,8 @@ class _Group:
     voxels: int
     live: dict = field(default_factory=dict)  # slot -> _Live
     busy: bool = False                        # a chunk is queued on the session stream
+    reading: int = 0                          # finished members still being read back
+    pooled: int = 0                           # the session's retired blocks [bytes], last seen
 
 
 def schedule_of(plan: MemberPlan, spec) -> tuple[int, int]:
@@ -184,8 +188,11 @@ class ContinuousBatcher:
         self.in_use = 0
         self.peak_in_use = 0  # high-water mark of admitted footprints (bytes)
         self.planners = ThreadPoolExecutor(max_workers=planners)
-        # host time of the serving loop by activity [s], and counts (what keeps the loop
-        # from refilling drained groups)
+        # read-back and retirement of finished members, off the serving loop
+        self.readers = ThreadPoolExecutor(max_workers=2)
+        # host time by activity [s] (admission and launches on the serving loop, read-back
+        # on the read-back threads), and counts (what keeps the loop from refilling
+        # drained groups)
         self.stats = {"admit_s": 0.0, "readback_s": 0.0, "launch_s": 0.0, "plan_wait_s": 0.0,
                       "admitted": 0, "chunks": 0, "idle_polls": 0}
 
@@ -223,12 +230,21 @@ class ContinuousBatcher:
 
     def close(self) -> None:
         self.planners.shutdown(wait=True)
+        self.readers.shutdown(wait=True)
         for g in self.groups.values():
             g.session.close()
         self.groups.clear()
 
     def _pooled(self) -> int:
-        return sum(x.session.pooled_bytes() for x in self.groups.values())
+        """Bytes of retired blocks the sessions hold (a session being read back counts with
+        its last value)."""
+        for x in self.groups.values():
+            if x.session.lock.acquire(blocking=False):
+                try:
+                    x.pooled = x.session.pooled_bytes()
+                finally:
+                    x.session.lock.release()
+        return sum(x.pooled for x in self.groups.values())
 
     def serve(self, requests: list[SolveRequest], t0: float | None = None,
               on_done=None) -> list[SolveResponse]:
@@ -257,6 +273,26 @@ class ContinuousBatcher:
             fut = self.planners.submit(self._plan, req)
             fut.add_done_callback(lambda f, r=req: planned.put((r, f)))
 
+        read_done: queue.Queue = queue.Queue()
+
+        def readback(g, lv):
+            """Read a finished member back and retire it (read-back thread). The session's
+            lock is held throughout: the serving loop skips the session meanwhile, so no
+            step sequence is being captured on it while its events are waited for."""
+            tr = time.perf_counter()
+            err, res, retire_err = None, None, None
+            with g.session.lock:
+                try:
+                    res = finish_result(g.session, lv.slot, lv.plan, lv.req, lv.total_steps,
+                                        lv.rec_steps, 0)
+                except Exception as exc:  # noqa: BLE001 - a failed solve fails alone
+                    err = f"{type(exc).__name__}: {exc}"
+                try:
+                    g.session.remove(lv.slot)
+                except Exception as exc:  # noqa: BLE001 - reported with the response
+                    retire_err = exc
+            read_done.put((g, lv, res, err, retire_err, time.perf_counter() - tr, now()))
+
         while len(responses) < n_total:
             # 1. arrivals -> planner pool
             t = now()
@@ -275,7 +311,8 @@ class ContinuousBatcher:
                     waiting.append((req, fut.result()))
                 except Exception as exc:  # noqa: BLE001 - a bad request fails alone
                     fail(req, exc)
-            # 3. groups whose chunk has drained: collect the members whose schedule is done
+            # 3. groups whose chunk has drained: hand the members whose schedule is done to
+            # the read-back threads (they hold their session slots until retired)
             finished = []
             for g in self.groups.values():
                 if g.busy:
@@ -284,13 +321,29 @@ class ContinuousBatcher:
                     g.busy = False
                 for slot in [s_ for s_, lv in g.live.items() if lv.done_steps >= lv.total_steps]:
                     finished.append((g, g.live.pop(slot)))
+            for g, lv in finished:
+                g.reading += 1
+                self.readers.submit(readback, g, lv)
+            # retired members: their memory and slots are free again
+            while True:
+                try:
+                    g, lv, res, err, retire_err, dt_read, t_done = read_done.get_nowait()
+                except queue.Empty:
+                    break
+                g.reading -= 1
+                self.in_use -= lv.plan.device_bytes()
+                self.stats["readback_s"] += dt_read
+                if retire_err is not None and err is None:
+                    err = f"{type(retire_err).__name__}: {retire_err}"
+                resp = SolveResponse(lv.req.rid, res, err, lv.req.arrival_s, lv.admit_s, t_done,
+                                     tuple(lv.req.grid.shape), lv.peak_batch)
+                responses.append(resp)
+                finished.append(None)  # progress this round
+                if on_done is not None:
+                    on_done(resp)
             # 4. admission in arrival order, by HBM budget and free slots; a running group
             # takes new members too (their upload does not wait for its chunk, and they
             # join its next one)
-            # members collected above still hold their session slots until step 6
-            retiring: dict = {}
-            for g_, _lv in finished:
-                retiring[id(g_)] = retiring.get(id(g_), 0) + 1
             still = []
             for req, (plan, total, rec, need) in waiting:
                 try:
@@ -301,11 +354,17 @@ class ContinuousBatcher:
                 pooled = self._pooled()
                 if self.in_use + need + pooled > self.hbm_budget and pooled:
                     for x in self.groups.values():
-                        if not x.busy:
-                            x.session.trim()  # retired blocks give way
+                        if not x.busy and x.session.lock.acquire(blocking=False):
+                            try:
+                                x.session.trim()  # retired blocks give way
+                            finally:
+                                x.session.lock.release()
                     pooled = self._pooled()
-                occupied = len(g.live) + retiring.get(id(g), 0)
+                occupied = len(g.live) + g.reading
                 if occupied < self.max_batch and self.in_use + need + pooled <= self.hbm_budget:
+                    if not g.session.lock.acquire(blocking=False):
+                        still.append((req, (plan, total, rec, need)))  # being read back
+                        continue
                     ta = time.perf_counter()
                     try:
                         slot = g.session.add(plan, max_periods=plan.eff_max + 4)
@@ -314,6 +373,7 @@ class ContinuousBatcher:
                         fail(req, exc)
                         continue
                     finally:
+                        g.session.lock.release()
                         self.stats["admit_s"] += time.perf_counter() - ta
                     self.stats["admitted"] += 1
                     g.live[slot] = _Live(req, light_plan(plan), slot, total, rec, now())
@@ -328,6 +388,8 @@ class ContinuousBatcher:
             for g in self.groups.values():
                 if g.busy or not g.live:
                     continue
+                if not g.session.lock.acquire(blocking=False):
+                    continue  # a member is being read back: refill on a later round
                 budget = self.chunk_steps * (256 ** 3)
                 k = max(1, budget // (g.voxels * len(g.live)))
                 k = min(k, min(lv.total_steps - lv.done_steps for lv in g.live.values()))
@@ -342,27 +404,13 @@ class ContinuousBatcher:
                         self._retire(g, slot, lv)
                         fail(lv.req, exc, lv.admit_s)
                     continue
+                finally:
+                    g.session.lock.release()
                 g.busy = True
                 launched = True
                 for lv in g.live.values():
                     lv.done_steps += k
                     lv.peak_batch = max(lv.peak_batch, len(g.live))
-            # 6. read back and retire the finished members while the chunks just queued run
-            for g, lv in finished:
-                tr = time.perf_counter()
-                err, res = None, None
-                try:
-                    res = finish_result(g.session, lv.slot, lv.plan, lv.req, lv.total_steps,
-                                        lv.rec_steps, 0)
-                except Exception as exc:  # noqa: BLE001 - a failed solve fails alone
-                    err = f"{type(exc).__name__}: {exc}"
-                self._retire(g, lv.slot, lv)
-                self.stats["readback_s"] += time.perf_counter() - tr
-                resp = SolveResponse(lv.req.rid, res, err, lv.req.arrival_s, lv.admit_s, now(),
-                                     tuple(lv.req.grid.shape), lv.peak_batch)
-                responses.append(resp)
-                if on_done is not None:
-                    on_done(resp)
             if not launched and not finished:
                 self.stats["idle_polls"] += 1
                 if not any(g.busy for g in self.groups.values()) and nxt < n_total \

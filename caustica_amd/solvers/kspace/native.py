@@ -14,6 +14,7 @@ from __future__ import annotations
 
 import contextlib
 import ctypes as C
+import threading
 
 import numpy as np
 
@@ -28,7 +29,14 @@ class KSpaceSession:
     """``scheme``: "reference" (the upstream collocated scheme) or "staggered" (the opt-in
     staggered / power-law / split-PML scheme); a session holds members of one scheme.
     ``graphs_default`` (class-wide) is whether new sessions replay captured HIP graphs of
-    the step sequence or launch every pass; the results are the same bits either way."""
+    the step sequence or launch every pass; the results are the same bits either way.
+
+    Threads: the calls that change the session's member table, block pool or graph cache
+    (add, remove, advance, trim, pooled_bytes) hold ``lock`` (reentrant). A thread that
+    retires finished members holds it across its read-back calls and ``remove``, so that
+    no step sequence is being captured on the session meanwhile; another thread can
+    admit and advance members of other sessions, and skip this one while it is taken
+    (``lock.acquire(blocking=False)``)."""
 
     graphs_default = True
 
@@ -44,6 +52,7 @@ class KSpaceSession:
         N.check(self._lib.cst_session_create_scheme(*self.padded3, int(max_members), int(device), code,
                                                    C.byref(h)))
         self._h = h
+        self.lock = threading.RLock()
         if not KSpaceSession.graphs_default:
             N.check(self._lib.cst_session_set_graphs(self._h, 0))
         self.max_members = int(max_members)
@@ -127,6 +136,12 @@ class KSpaceSession:
             max_periods = plan.eff_max + 4
         d.max_periods = int(max_periods)
         slot = C.c_int(-1)
+        with self.lock:
+            self._add_locked(plan, d, maps, arr, slot)
+        self._plans[slot.value] = plan
+        return slot.value
+
+    def _add_locked(self, plan, d, maps, arr, slot) -> None:
         if plan.scheme == "staggered":
             g = plan.stg
             sd = N.StaggeredDesc()
@@ -145,21 +160,22 @@ class KSpaceSession:
             with D.borrow(maps[0]) as m0, D.borrow(maps[1]) as m1, D.borrow(maps[2]) as m2, \
                     D.borrow(maps[3]) as m3:
                 N.check(self._lib.cst_member_add_dl(self._h, C.byref(d), m0, m1, m2, m3, C.byref(slot)))
-        self._plans[slot.value] = plan
-        return slot.value
 
     def remove(self, slot: int) -> None:
-        N.check(self._lib.cst_member_remove(self._h, int(slot)))
-        self._plans.pop(slot, None)
+        with self.lock:
+            N.check(self._lib.cst_member_remove(self._h, int(slot)))
+            self._plans.pop(slot, None)
 
     def trim(self) -> int:
         n = C.c_size_t(0)
-        N.check(self._lib.cst_session_trim(self._h, C.byref(n)))
+        with self.lock:
+            N.check(self._lib.cst_session_trim(self._h, C.byref(n)))
         return int(n.value)
 
     def pooled_bytes(self) -> int:
         n = C.c_size_t(0)
-        N.check(self._lib.cst_session_pooled_bytes(self._h, C.byref(n)))
+        with self.lock:
+            N.check(self._lib.cst_session_pooled_bytes(self._h, C.byref(n)))
         return int(n.value)
 
     def reset(self, slot: int) -> None:
@@ -180,7 +196,8 @@ class KSpaceSession:
     def advance(self, slots, n_steps: int) -> None:
         slots = [int(s) for s in slots]
         arr = (C.c_int * len(slots))(*slots)
-        N.check(self._lib.cst_advance(self._h, len(slots), arr, int(n_steps)))
+        with self.lock:
+            N.check(self._lib.cst_advance(self._h, len(slots), arr, int(n_steps)))
 
     def sync(self) -> None:
         N.check(self._lib.cst_sync(self._h))

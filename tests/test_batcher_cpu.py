@@ -8,6 +8,9 @@ admission, a diverged field at read-back. Admission into a group whose chunk is 
 flight is allowed: the new member joins the next chunk.
 """
 
+import threading
+import time
+
 import numpy as np
 import pytest
 
@@ -20,6 +23,8 @@ class FakeSession:
     fail_add_for: set = set()      # member plans (by id) whose add() raises
     nan_for: set = set()           # plans whose peaks come back non-finite
     busy_polls = 0                 # idle() reports busy this many times per chunk
+    slow_read: dict = {}           # padded shape -> seconds a read-back takes
+    events: list = []              # (time, what, padded shape)
 
     def __init__(self, padded3, max_members=1, device=0):
         self.padded3 = tuple(padded3)
@@ -28,6 +33,7 @@ class FakeSession:
         self.next_slot = 0
         self.log = []
         self._busy = 0
+        self.lock = threading.RLock()
         FakeSession.instances.append(self)
 
     def add(self, plan, max_periods=None):
@@ -46,6 +52,7 @@ class FakeSession:
 
     def advance(self, slots, k):
         self.log.append(("advance", tuple(slots), k))
+        FakeSession.events.append((time.perf_counter(), "advance", self.padded3))
         self._busy = FakeSession.busy_polls
 
     def idle(self):
@@ -55,6 +62,9 @@ class FakeSession:
         return True
 
     def peaks(self, slot, first, count):
+        FakeSession.events.append((time.perf_counter(), "read start", self.padded3))
+        time.sleep(FakeSession.slow_read.get(self.padded3, 0.0))
+        FakeSession.events.append((time.perf_counter(), "read end", self.padded3))
         v = np.nan if id(self.plans[slot]) in FakeSession.nan_for else 1.0
         return np.full(count, v, np.float32)
 
@@ -82,6 +92,8 @@ def fake_session(monkeypatch):
     FakeSession.fail_add_for = set()
     FakeSession.nan_for = set()
     FakeSession.busy_polls = 0
+    FakeSession.slow_read = {}
+    FakeSession.events = []
     monkeypatch.setattr(B, "KSpaceSession", FakeSession)
     yield
 
@@ -208,3 +220,25 @@ def test_slots_of_retiring_members_are_not_reused_early():
         b.close()
     assert [r.error for r in resp if r.error] == []
     assert len(resp) == 9
+
+
+@pytest.mark.timeout(60)
+def test_other_shapes_advance_while_a_member_is_read_back():
+    """Read-back runs off the serving loop: while a finished member of one shape is read
+    back (slowly here), members of another shape keep receiving chunks."""
+    rng = np.random.default_rng(5)
+    a = workloads.serving_request(0, rng, 0.0, shape=(48, 48, 64), solver="linear")
+    b = [workloads.serving_request(i, rng, 0.0, shape=(64, 64, 64), solver="linear") for i in (1, 2)]
+    FakeSession.slow_read = {(48, 48, 64): 0.5}
+    FakeSession.busy_polls = 20
+    bat = B.ContinuousBatcher(hbm_budget=1 << 40, max_batch=4, planners=2, chunk_steps=1)
+    try:
+        resp = bat.serve([a] + b)
+    finally:
+        bat.close()
+    assert sorted(r.rid for r in resp) == [0, 1, 2] and all(r.error is None for r in resp)
+    t0 = next(t for t, what, shp in FakeSession.events if what == "read start" and shp == (48, 48, 64))
+    t1 = next(t for t, what, shp in FakeSession.events if what == "read end" and shp == (48, 48, 64))
+    during = [t for t, what, shp in FakeSession.events if what == "advance" and shp == (64, 64, 64) and t0 < t < t1]
+    assert during, "the other shape's group got no chunk while the member was read back"
+    assert bat.in_use == 0
