@@ -513,6 +513,10 @@ struct RhoLds {
 // grid's axes, rho = sum_a rho_a. ABS = false: p = c^2 rho (+ c^2 m (beta - 1) rho^2 / rho0),
 // peak, record and z-R2C of p into sp. ABS = true: rho goes to p's buffer for k_szabs and
 // dt rho0 sum_a d_a and rho are z-R2C'd into sg0, sg1 for the fractional Laplacians.
+// The components are streamed one at a time (load, update, source, store), so that only
+// one of them, the running sum and the weight are held in registers; the source adds to
+// each component right after its update, which is where the oracle's order puts it too
+// (the update of one component does not read the others').
 template <int M, bool ABS>
 __global__ void __launch_bounds__((ZPlan<M, 0>::THREADS))
     k_szrho(StepArgs a, const float2* __restrict__ twg, const float2* __restrict__ rtg) {
@@ -530,15 +534,15 @@ __global__ void __launch_bounds__((ZPlan<M, 0>::THREADS))
   const long e0 = line0 * M;
   const bool nonlin = m.stg.nl2dt != nullptr;
   const int src_axes = m.stg.src_axes;
+  const gptr<float2> rhog[3] = {G(reinterpret_cast<float2*>(m.stg.rho[0])) + e0,
+                                G(reinterpret_cast<float2*>(m.stg.rho[1])) + e0,
+                                G(reinterpret_cast<float2*>(m.stg.rho[2])) + e0};
+  const gptr<const float2> dto0g = G(reinterpret_cast<const float2*>(m.stg.dtrho0)) + e0;
   ZStage<M, 0> stg;
   z_stage_load<M, 0>(twg, rtg, m.stg.pml, a.s, line0, nl, stg);
-  float2 sr[CN::EPT], srn;
+  float2 sr[CN::EPT], srn, fac[CN::EPT];
   z_spec_load<M, 0>(G(m.sg[0]), line0, nl, a.s, sr, srn);
-  float2 r0[CN::EPT], r1[CN::EPT], r2[CN::EPT], fac[CN::EPT];
-  z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.rho[0])) + e0, nl, r0);
-  z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.rho[1])) + e0, nl, r1);
-  z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.rho[2])) + e0, nl, r2);
-  z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.dtrho0)) + e0, nl, fac);
+  z_vol_load<M, 0>(dto0g, nl, fac);
   const gptr<const int> sptr = G(m.src_ptr);
   const int sp_l = sptr[line0 + ((int)threadIdx.x <= nl ? (int)threadIdx.x : nl)];
   z_stage_put<M, 0>(stg, a.s, line0, nl, s);
@@ -552,23 +556,35 @@ __global__ void __launch_bounds__((ZPlan<M, 0>::THREADS))
   z_spec_put<M, 0>(nl, sr, srn, s);
   __syncthreads();
   z_spec_load<M, 0>(G(m.sg[1]), line0, nl, a.s, sr, srn);
-  float2 dto0[ABS ? CN::EPT : 1];  // dt rho0 (absorbing: the weight of sum_a d_a)
-  if constexpr (ABS) {
-#pragma unroll
-    for (int j = 0; j < CN::EPT; ++j) dto0[j] = fac[j];
-  }
   if (nonlin) {
-    float2 nl2[CN::EPT];
-    z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.nl2dt)) + e0, nl, nl2);
+    // the weight takes the density before this step's update: sum the old components
+    float2 os[CN::EPT], o[CN::EPT];
+    z_vol_load<M, 0>(rhog[0], nl, os);
+#pragma unroll
+    for (int ax = 1; ax < 3; ++ax) {
+      z_vol_load<M, 0>(rhog[ax], nl, o);
+#pragma unroll
+      for (int j = 0; j < CN::EPT; ++j) {
+        os[j].x += o[j].x;
+        os[j].y += o[j].y;
+      }
+    }
+    z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.nl2dt)) + e0, nl, o);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
-      fac[j].x += nl2[j].x * ((r0[j].x + r1[j].x) + r2[j].x);
-      fac[j].y += nl2[j].y * ((r0[j].y + r1[j].y) + r2[j].y);
+      fac[j].x += o[j].x * os[j].x;
+      fac[j].y += o[j].y * os[j].y;
     }
   }
-  float2 ds[CN::EPT];  // sum_a d_a (absorbing members)
-  // axis a: C2R of its divergence term (in s.a), then rho_a = s_a (s_a rho_a - fac d_a)
-  auto update = [&](int ax, float2 (&r)[CN::EPT]) {
+  const gptr<const int> srcz = G(m.src_z);
+  const gptr<const float2> srcab = G(m.src_ab);
+  float2 rsum[CN::EPT];               // rho = (rho_0 + rho_1) + rho_2
+  float2 ds[ABS ? CN::EPT : 1];       // sum_a d_a (absorbing members)
+  // axis a: C2R of its divergence term (in s.a), rho_a = s_a (s_a rho_a - fac d_a), the
+  // source, the store of rho_a
+  auto update = [&](int ax) {
+    float2 r[CN::EPT];
+    z_vol_load<M, 0>(rhog[ax], nl, r);
     z_c2r<M, 0>(ll, t, lvalid, s);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
@@ -584,8 +600,26 @@ __global__ void __launch_bounds__((ZPlan<M, 0>::THREADS))
         fx = s.sz[mm].x;
         fy = s.sz[mm].y;
       }
-      r[j].x = fx * (fx * r[j].x - fac[j].x * d.x);
-      r[j].y = fy * (fy * r[j].y - fac[j].y * d.y);
+      float2 rv = make_float2(fx * (fx * r[j].x - fac[j].x * d.x), fy * (fy * r[j].y - fac[j].y * d.y));
+      if ((src_axes >> ax) & 1) {
+        // duplicates are refused at admission: at most one entry per element
+        for (int e = s.sptr[l]; e < s.sptr[l + 1]; ++e) {
+          const int zz = srcz[e];
+          if ((zz >> 1) == mm) {
+            const float2 w = srcab[e];
+            const float v = sm.amp * (w.x * sm.sin_t - w.y * sm.cos_t);
+            if (zz & 1) rv.y += v;
+            else rv.x += v;
+          }
+        }
+      }
+      if (l0 < nl) rhog[ax][l * M + mm] = rv;
+      if (ax == 0) {
+        rsum[j] = rv;
+      } else {
+        rsum[j].x += rv.x;
+        rsum[j].y += rv.y;
+      }
       if constexpr (ABS) {
         if (ax == 0) ds[j] = d;
         else {
@@ -596,66 +630,32 @@ __global__ void __launch_bounds__((ZPlan<M, 0>::THREADS))
     }
     __syncthreads();  // s.a consumed
   };
-  update(0, r0);
+  update(0);
   z_spec_put<M, 0>(nl, sr, srn, s);
   __syncthreads();
   z_spec_load<M, 0>(G(m.sg[2]), line0, nl, a.s, sr, srn);
-  update(1, r1);
+  update(1);
   z_spec_put<M, 0>(nl, sr, srn, s);
   __syncthreads();
-  float2 c2[CN::EPT], bh[CN::EPT];
-  if constexpr (!ABS) {
-    z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.c2)) + e0, nl, c2);
-    if (nonlin) z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.c2bh)) + e0, nl, bh);
-  }
-  update(2, r2);
-  // mass source (duplicates are refused at admission, so at most one entry per element)
-  const gptr<const int> srcz = G(m.src_z);
-  const gptr<const float2> srcab = G(m.src_ab);
-#pragma unroll
-  for (int j = 0; j < CN::EPT; ++j) {
-    const int l = ZElem<M, 0>::line(j), mm = ZElem<M, 0>::pos(j, l);
-    if (l < nl) {
-      for (int e = s.sptr[l]; e < s.sptr[l + 1]; ++e) {
-        const int zz = srcz[e];
-        if ((zz >> 1) == mm) {
-          const float2 w = srcab[e];
-          const float v = sm.amp * (w.x * sm.sin_t - w.y * sm.cos_t);
-          if (zz & 1) {
-            if (src_axes & 1) r0[j].y += v;
-            if (src_axes & 2) r1[j].y += v;
-            if (src_axes & 4) r2[j].y += v;
-          } else {
-            if (src_axes & 1) r0[j].x += v;
-            if (src_axes & 2) r1[j].x += v;
-            if (src_axes & 4) r2[j].x += v;
-          }
-        }
-      }
-    }
-  }
-  const gptr<float2> rho0g = G(reinterpret_cast<float2*>(m.stg.rho[0])) + e0;
-  const gptr<float2> rho1g = G(reinterpret_cast<float2*>(m.stg.rho[1])) + e0;
-  const gptr<float2> rho2g = G(reinterpret_cast<float2*>(m.stg.rho[2])) + e0;
   const gptr<float2> pg = G(reinterpret_cast<float2*>(m.p)) + e0;
   if constexpr (!ABS) {
+    float2 c2[CN::EPT], bh[CN::EPT];
+    z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.c2)) + e0, nl, c2);
+    if (nonlin) z_vol_load<M, 0>(G(reinterpret_cast<const float2*>(m.stg.c2bh)) + e0, nl, bh);
+    update(2);
     PSink sink;
     sink.init(m, sm);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
       const int l = ZElem<M, 0>::line(j), mm = ZElem<M, 0>::pos(j, l);
       if (l < nl) {
-        const int i = l * M + mm;
-        rho0g[i] = r0[j];
-        rho1g[i] = r1[j];
-        rho2g[i] = r2[j];
-        const float2 rho = make_float2((r0[j].x + r1[j].x) + r2[j].x, (r0[j].y + r1[j].y) + r2[j].y);
+        const float2 rho = rsum[j];
         float2 pv = make_float2(c2[j].x * rho.x, c2[j].y * rho.y);
         if (nonlin) {
           pv.x += bh[j].x * rho.x * rho.x;
           pv.y += bh[j].y * rho.y * rho.y;
         }
-        pg[i] = pv;
+        pg[l * M + mm] = pv;
         sink.put(s.lx[l], s.ly[l], mm, pv);
         s.a[l * Z::LSA + mm] = pv;
       }
@@ -664,18 +664,16 @@ __global__ void __launch_bounds__((ZPlan<M, 0>::THREADS))
     __syncthreads();
     z_r2c<M, 0>(ll, t, lvalid, s, G(m.sp) + s.row[lvalid ? ll : 0], G(m.sp) + a.s.so + line0 + (lvalid ? ll : 0));
   } else {
+    update(2);
+    // dt rho0, the weight of sum_a d_a: the weight itself on linear members, reread on
+    // nonlinear ones (whose weight carries the density term)
+    if (nonlin) z_vol_load<M, 0>(dto0g, nl, fac);
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
       const int l = ZElem<M, 0>::line(j), mm = ZElem<M, 0>::pos(j, l);
       if (l < nl) {
-        const int i = l * M + mm;
-        rho0g[i] = r0[j];
-        rho1g[i] = r1[j];
-        rho2g[i] = r2[j];
-        const float2 rho = make_float2((r0[j].x + r1[j].x) + r2[j].x, (r0[j].y + r1[j].y) + r2[j].y);
-        pg[i] = rho;
-        r0[j] = rho;  // kept for the second transform
-        s.a[l * Z::LSA + mm] = make_float2(dto0[j].x * ds[j].x, dto0[j].y * ds[j].y);
+        pg[l * M + mm] = rsum[j];
+        s.a[l * Z::LSA + mm] = make_float2(fac[j].x * ds[j].x, fac[j].y * ds[j].y);
       }
     }
     __syncthreads();
@@ -685,7 +683,7 @@ __global__ void __launch_bounds__((ZPlan<M, 0>::THREADS))
 #pragma unroll
     for (int j = 0; j < CN::EPT; ++j) {
       const int l = ZElem<M, 0>::line(j), mm = ZElem<M, 0>::pos(j, l);
-      if (l < nl) s.a[l * Z::LSA + mm] = r0[j];
+      if (l < nl) s.a[l * Z::LSA + mm] = rsum[j];
     }
     __syncthreads();
     z_r2c<M, 0>(ll, t, lvalid, s, G(m.sg[1]) + s.row[lvalid ? ll : 0],
