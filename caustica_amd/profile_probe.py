@@ -120,9 +120,11 @@ def summarize_sq(a_dir: str, b_dir: str) -> str:
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2", "pennes"],
-                    help="k-space workload, or pennes: the thermal benchmark's 256^3 tissue "
-                         "heated by a seeded random source, --steps Pennes steps with the dose")
+    ap.add_argument("--workload", default="cfg1", choices=["cfg0", "cfg1", "cfg2", "staggered", "pennes"],
+                    help="k-space workload; staggered: the 256^3 medium under the opt-in "
+                         "staggered power-law scheme (y = 1.1); or pennes: the thermal "
+                         "benchmark's 256^3 tissue heated by a seeded random source, --steps "
+                         "Pennes steps with the dose")
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--variant", type=int, default=0, help="z-pass thread layout (0 or 1)")
     ap.add_argument("--zu-variant", type=int, help="velocity-pass layout (0, 1 or 2; default --variant)")
@@ -163,10 +165,17 @@ def main() -> int:
         print(f"[probe] peak {float(r.temperature_max.max()):.3f} C", flush=True)
         return 0
     wl = {"cfg0": workloads.cfg0_water_bowl, "cfg1": workloads.cfg1_hetero_linear,
-          "cfg2": workloads.cfg2_westervelt}[args.workload]()
-    plan = plan_member(wl.grid, wl.medium, wl.source, wl.spec, harmonics=wl.harmonics,
-                       reference_point=wl.reference_point, nonlinear=wl.solver == "westervelt")
-    sess = KSpaceSession(plan.padded3, 1)
+          "cfg2": workloads.cfg2_westervelt, "staggered": workloads.cfg1_hetero_linear}[args.workload]()
+    if args.workload == "staggered":
+        from caustica_amd.solvers.kspace.staggered import plan_staggered
+
+        plan = plan_staggered(wl.grid, wl.medium, wl.source, wl.spec, harmonics=wl.harmonics,
+                              reference_point=wl.reference_point, alpha_power=1.1)
+    else:
+        plan = plan_member(wl.grid, wl.medium, wl.source, wl.spec, harmonics=wl.harmonics,
+                           reference_point=wl.reference_point, nonlinear=wl.solver == "westervelt")
+    sess = KSpaceSession(plan.padded3, 1, scheme=plan.scheme)
+    sess.set_graphs(False)  # counters are collected per dispatch of directly launched kernels
     zu = args.variant if args.zu_variant is None else args.zu_variant
     zp = args.variant if args.zp_variant is None else args.zp_variant
     sess.set_variant(2, zu)
