@@ -143,20 +143,36 @@ def ramp_steps(dt: float, f0: float, ramp_periods: float) -> int:
 
 
 def hip_inventory(padded3, *, nonlinear: bool, n_harmonics: int, rec_elems: int,
-                  n_src: int, max_periods: int, spp: int = 1, ramp_n: int = 0) -> MemoryModel:
-    """Bytes of one member on the hip engine, piece by piece as the library carves them."""
+                  n_src: int, max_periods: int, spp: int = 1, ramp_n: int = 0,
+                  staggered_maps: int = 0) -> MemoryModel:
+    """Bytes of one member on the hip engine, piece by piece as the library carves them.
+    ``staggered_maps`` > 0: a member of the opt-in staggered scheme with that many
+    coefficient maps (5, + 2 absorbing, + 2 nonlinear), its split density and its PML and
+    shift-factor tables in place of the reference maps and sponge."""
     nx, ny, nz = (int(v) for v in padded3)
     nzh = nz // 2 + 1
     ms = (nz // 2 + 15) // 16 * 16
     P = nx * ny * nz
     H = nx * (ny * ms + 16) + (nx * ny + 15) // 16 * 16  # rows + Nyquist side plane
     ns = max(int(n_src), 1)
+    if staggered_maps:
+        pst = (nz + nx + ny + 63) // 64 * 64
+        scheme = {
+            "split density (rho_0-2)": _al(3 * P * 4),
+            "property maps": _al(int(staggered_maps) * P * 4),
+            "PML tables": _al(4 * pst * 4),
+            "shift factors": _al((nx + ny + nzh) * 8),
+        }
+    else:
+        scheme = {
+            "property maps": _al((4 if nonlinear else 3) * P * 4),
+            "sponge profiles": _al((nz + nx + ny) * 4),
+        }
     b = {
         "pressure (p)": _al(P * 4),
         "half spectra (sp, sg0-2, velocity state su0-2)": _al(7 * H * 8),
-        "property maps": _al((4 if nonlinear else 3) * P * 4),
+        **scheme,
         "k tables": _al(2 * (nx + ny + nzh) * 4),
-        "sponge profiles": _al((nz + nx + ny) * 4),
         "source table": _al((nx * ny + 1) * 4) + _al(ns * 4) + _al(ns * 8),
         "record buffers": _al(2 * n_harmonics * rec_elems * 8) + _al(rec_elems * 4),
         "convergence peaks": _al(max(int(max_periods), 1) * 64 * 16 * 4),
@@ -169,11 +185,13 @@ def hip_inventory(padded3, *, nonlinear: bool, n_harmonics: int, rec_elems: int,
 def hip_memory(plan: MemberPlan, max_periods: int | None = None) -> MemoryModel:
     """``hip_inventory`` of a planned member (the same ``max_periods`` default as
     ``KSpaceSession.add``)."""
+    nmaps = sum(m is not None for m in plan.stg["maps"]) if plan.scheme == "staggered" else 0
     return hip_inventory(plan.padded3, nonlinear=plan.beta2_dt is not None,
                          n_harmonics=len(plan.harmonics), rec_elems=int(np.prod(plan.rec_shape)),
                          n_src=int(plan.src_idx3.shape[0]),
                          max_periods=plan.eff_max + 4 if max_periods is None else max_periods,
-                         spp=plan.spp, ramp_n=ramp_steps(plan.dt, plan.f0, plan.ramp_periods))
+                         spp=plan.spp, ramp_n=ramp_steps(plan.dt, plan.f0, plan.ramp_periods),
+                         staggered_maps=nmaps)
 
 
 # ------------------------------------------------------------------- time

@@ -44,6 +44,28 @@ def test_hip_inventory_is_the_allocation(solver, harmonics, region):
         assert s.pooled_bytes() == want.total_bytes
 
 
+@pytest.mark.parametrize("solver,alpha_zero", [("linear", True), ("westervelt", False)])
+def test_hip_inventory_of_a_staggered_member_is_the_allocation(solver, alpha_zero):
+    """The same for the opt-in staggered scheme (split density, its own maps and tables)."""
+    from caustica_amd.solvers.kspace.staggered import plan_staggered
+
+    req = _req(14, solver)
+    medium = req.medium
+    if alpha_zero:
+        medium = type(medium)(alpha=np.zeros_like(medium.alpha), rho=medium.rho, c=medium.c,
+                              beta=medium.beta)
+    plan = plan_staggered(req.grid, medium, req.source, req.spec, harmonics=req.harmonics,
+                          nonlinear=solver == "westervelt", alpha_power=1.2)
+    assert plan.stg["absorbing"] == (not alpha_zero)
+    want = planner.hip_memory(plan)
+    print(f"[planner] staggered {solver}: {want.total_bytes} B", flush=True)
+    with KSpaceSession(plan.padded3, 1, scheme="staggered") as s:
+        slot = s.add(plan)
+        s.remove(slot)
+        assert s.pooled_bytes() == want.total_bytes
+    assert abs(plan.device_bytes() - want.total_bytes) <= (1 << 20) + 256 * 32
+
+
 def test_steps_bracketed_by_estimate():
     req = _req(12, "linear")
     est = planner.estimate(req.grid, req.medium, req.source, req.spec, solver="linear",
