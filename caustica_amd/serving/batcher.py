@@ -167,6 +167,9 @@ class ContinuousBatcher:
     and small batches take more steps per chunk), so every chunk is a few milliseconds
     of device work and the host revisits a group only when its stream has drained.
 
+    max_launch_voxels caps the voxels one launch advances in a shape group (see
+    __init__); requests beyond the cap wait for a slot of their shape.
+
     A request that cannot be served fails alone, as an error response: planning errors
     (a schedule that is not pinned, a shape without kernels), a footprint larger than
     the whole budget, an allocation or upload failure at admission, a diverged field at
@@ -174,7 +177,7 @@ class ContinuousBatcher:
     """
 
     def __init__(self, hbm_budget: int | None = None, max_batch: int = 16, chunk_steps: int = 48,
-                 planners: int = 4, device: int = 0):
+                 planners: int = 4, device: int = 0, max_launch_voxels: int | None = 6 << 20):
         from caustica_amd import _native as N
 
         self.device = device
@@ -183,6 +186,14 @@ class ContinuousBatcher:
             hbm_budget = int(0.8 * free)
         self.hbm_budget = int(hbm_budget)
         self.max_batch = int(max_batch)
+        # members of one shape advanced together are capped at this many voxels per launch
+        # (at least one member; None: max_batch members of any size). The default, three
+        # 128^3 grids, lets the small shapes share launches and runs 192^3 and larger one
+        # member at a time: with shapes overlapping on their own streams, larger batches
+        # lost more to cache reuse than they gained (a 96-request mixed queue at 32 req/s:
+        # 5.35 solves/s uncapped, 5.50 at 16.8 M voxels, 5.60-5.65 at 2.1-6.3 M), and the
+        # first of several large requests no longer waits for the others' steps
+        self.max_launch_voxels = max_launch_voxels
         self.chunk_steps = int(chunk_steps)
         self.groups: dict[tuple, _Group] = {}
         self.in_use = 0
@@ -361,7 +372,10 @@ class ContinuousBatcher:
                                 x.session.lock.release()
                     pooled = self._pooled()
                 occupied = len(g.live) + g.reading
-                if occupied < self.max_batch and self.in_use + need + pooled <= self.hbm_budget:
+                cap = self.max_batch
+                if self.max_launch_voxels is not None:
+                    cap = max(1, min(cap, int(self.max_launch_voxels) // g.voxels))
+                if occupied < cap and self.in_use + need + pooled <= self.hbm_budget:
                     if not g.session.lock.acquire(blocking=False):
                         still.append((req, (plan, total, rec, need)))  # being read back
                         continue
