@@ -153,6 +153,44 @@ __device__ __forceinline__ void in_finish(int t, At at,
   }
 }
 
+// Stage "in" finish shared by two threads per A-point DFT, for lines whose T threads
+// number twice the B stage-2 DFTs (T = 2 B, A even; e.g. M = 128 = 16 x 8 on 16 threads,
+// where the plain finish leaves half the lanes idle). Threads k2 and k2 + B take column
+// k2 as one radix-2 decimation-in-frequency step: x[n] + x[n + A/2] (even outputs) and
+// (x[n] - x[n + A/2]) W_A^n (odd outputs), then an (A/2)-point DFT. Thread t = k2 + h B
+// ends with outputs k1 = 2 m + h, m < A/2, i.e. bins k2 + B k1 = t + T m.
+template <int L, int T, int AA>
+struct PairFinish {
+  using S = Split<L, T, AA>;
+  static constexpr bool OK = 2 * S::B == T && S::A % 2 == 0 && S::A >= 4;
+  static constexpr int H = S::A / 2;
+};
+
+template <int DIR, int A, int N, class At>
+__device__ __forceinline__ float2 pair_in(At at, int k2, float sg, bool odd) {
+  const float2 x0 = *at(N + A * k2), x1 = *at(N + A / 2 + A * k2);
+  // x0 + sg x1 with sg = +-1 rounds exactly as the sum or the difference
+  const float2 v = make_float2(fmaf(sg, x1.x, x0.x), fmaf(sg, x1.y, x0.y));
+  return odd ? twiddle_const<DIR, N, A>(v) : v;
+}
+
+template <int L, int T, int DIR, int AA, class At, int... N>
+__device__ __forceinline__ void in_finish_pair_(int t, At at, float2 (&Y)[PairFinish<L, T, AA>::H],
+                                                std::integer_sequence<int, N...>) {
+  using S = Split<L, T, AA>;
+  const bool odd = t >= S::B;
+  const int k2 = odd ? t - S::B : t;
+  const float sg = odd ? -1.f : 1.f;
+  ((Y[N] = pair_in<DIR, S::A, N>(at, k2, sg, odd)), ...);
+  Dft<S::A / 2, DIR>::run(Y);
+}
+
+template <int L, int T, int DIR, int AA, class At>
+__device__ __forceinline__ void in_finish_pair(int t, At at, float2 (&Y)[PairFinish<L, T, AA>::H]) {
+  static_assert(PairFinish<L, T, AA>::OK, "pair finish needs T = 2 B and A even");
+  in_finish_pair_<L, T, DIR, AA>(t, at, Y, std::make_integer_sequence<int, PairFinish<L, T, AA>::H>{});
+}
+
 // stage "out": input X[k2 + B k1] as X[r][k1]  ->  natural x[m2 + A j] as V[r][j], m2 = t + T r
 template <int L, int T, int DIR, class At>
 __device__ __forceinline__ void out_compute(int t, float2 (&X)[Split<L, T>::R2][Split<L, T>::A], At at,
@@ -984,6 +1022,39 @@ __device__ __forceinline__ void z_vol_load(gptr<const float2> v, int nl,
   }
 }
 
+// Second stage of a z-pass line transform: exchange buffer le -> natural order in la.
+// Shapes with twice as many threads per line as stage-2 DFTs share each DFT between two
+// threads (PairFinish), the others run the plain finish.
+template <int M, int ZV, int DIR>
+__device__ __forceinline__ void z_finish_store(int t, bool lvalid, float2* le, float2* la) {
+  using Z = ZPlan<M, ZV>;
+  using S = Split<M, Z::ZT, Z::A>;
+  if constexpr (PairFinish<M, Z::ZT, Z::A>::OK) {
+    constexpr int H = PairFinish<M, Z::ZT, Z::A>::H;
+    float2 Y[H];
+    if (lvalid) in_finish_pair<M, Z::ZT, DIR, Z::A>(t, LinePadAt<S::A>{le}, Y);
+    if constexpr (Z::ALIAS) __syncthreads();  // e consumed before a (the same array) is written
+    if (lvalid) {
+#pragma unroll
+      for (int m = 0; m < H; ++m) la[t + Z::ZT * m] = Y[m];
+    }
+  } else {
+    float2 X[S::R2][S::A];
+    if (lvalid) in_finish<M, Z::ZT, DIR, Z::A>(t, LinePadAt<S::A>{le}, X);
+    if constexpr (Z::ALIAS) __syncthreads();  // e consumed before a (the same array) is written
+    if (lvalid) {
+#pragma unroll
+      for (int r = 0; r < S::R2; ++r) {
+        const int k2 = t + Z::ZT * r;
+        if (k2 < S::B) {
+#pragma unroll
+          for (int k1 = 0; k1 < S::A; ++k1) la[k2 + S::B * k1] = X[r][k1];
+        }
+      }
+    }
+  }
+}
+
 // C2R of every staged line: natural buffer a -> real line (as M float2) back in a.
 template <int M, int ZV>
 __device__ __forceinline__ void z_c2r(int ll, int t, bool lvalid, ZLds<M, ZV>& s) {
@@ -996,19 +1067,7 @@ __device__ __forceinline__ void z_c2r(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   __syncthreads();  // a consumed
   if (lvalid) in_compute<M, Z::ZT, +1, true, Z::A>(t, V, LinePadAt<S::A>{le}, s.tw);
   __syncthreads();
-  float2 X[S::R2][S::A];
-  if (lvalid) in_finish<M, Z::ZT, +1, Z::A>(t, LinePadAt<S::A>{le}, X);
-  if constexpr (Z::ALIAS) __syncthreads();  // e consumed before a (the same array) is written
-  if (lvalid) {
-#pragma unroll
-    for (int r = 0; r < S::R2; ++r) {
-      const int k2 = t + Z::ZT * r;
-      if (k2 < S::B) {
-#pragma unroll
-        for (int k1 = 0; k1 < S::A; ++k1) la[k2 + S::B * k1] = X[r][k1];
-      }
-    }
-  }
+  z_finish_store<M, ZV, +1>(t, lvalid, le, la);
   __syncthreads();
 }
 
@@ -1026,19 +1085,7 @@ __device__ __forceinline__ void z_r2c(int ll, int t, bool lvalid, ZLds<M, ZV>& s
   if constexpr (Z::ALIAS) __syncthreads();  // a consumed before e (the same array) is written
   if (lvalid) in_compute<M, Z::ZT, -1, true, Z::A>(t, V, LinePadAt<S::A>{le}, s.tw);
   __syncthreads();  // e written, a consumed
-  float2 X[S::R2][S::A];
-  if (lvalid) in_finish<M, Z::ZT, -1, Z::A>(t, LinePadAt<S::A>{le}, X);
-  if constexpr (Z::ALIAS) __syncthreads();  // e consumed before a is written
-  if (lvalid) {
-#pragma unroll
-    for (int r = 0; r < S::R2; ++r) {
-      const int k2 = t + Z::ZT * r;
-      if (k2 < S::B) {
-#pragma unroll
-        for (int k1 = 0; k1 < S::A; ++k1) la[k2 + S::B * k1] = X[r][k1];
-      }
-    }
-  }
+  z_finish_store<M, ZV, -1>(t, lvalid, le, la);
   __syncthreads();
   if (lvalid) {
     for (int k = t; k < M; k += Z::ZT) dst_row[k] = r2c_post<M>(la, k, s.rt);
