@@ -601,9 +601,10 @@ struct KxStage {
 };
 
 // Waves per SIMD requested for k_xgrad: four (128 VGPRs) where the compiler fits the
-// kernel without spilling (the default tile width), else no constraint.
+// kernel without spilling (the default tile width), else no constraint. At L = 256 four
+// waves spill four VGPRs to scratch; three waves without the spill run 1.5 % faster.
 constexpr int xgrad_waves(int L, int TV) {
-  return (TV == 0 && L <= 256 && L != 144 && L != 200 && L != 240) ? 4 : 1;
+  return (TV == 0 && L < 256 && L != 144 && L != 200 && L != 240) ? 4 : 1;
 }
 
 // gradient: sg[0] = IFFT_x(kx_d i kappa / N FFT_x(sp)), sg[1] = Q = IFFT_x(i kappa / N
