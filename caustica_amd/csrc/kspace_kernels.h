@@ -839,11 +839,13 @@ struct ZCount {
 // (x, y) of line line0 + l of a block. Lines of a block are consecutive, so when the
 // block holds at most ny lines they cross at most one x plane: one compare instead of a
 // per-thread integer division. x0, y0 (the block's first line) are block-uniform.
-template <int LPB>
-__device__ __forceinline__ void line_xy(int x0, int y0, int l, const Shape& sh, int& x, int& y) {
+// NARROW: the caller has established LPB <= ny (block-uniform), so the test is not
+// repeated per element.
+template <int LPB, bool NARROW>
+__device__ __forceinline__ void line_xy_(int x0, int y0, int l, const Shape& sh, int& x, int& y) {
   y = y0 + l;
   x = x0;
-  if (LPB <= sh.ny) {
+  if constexpr (NARROW) {
     const bool w = y >= sh.ny;
     y -= w ? sh.ny : 0;
     x += w ? 1 : 0;
@@ -851,6 +853,11 @@ __device__ __forceinline__ void line_xy(int x0, int y0, int l, const Shape& sh, 
     x += y / sh.ny;
     y -= (y / sh.ny) * sh.ny;
   }
+}
+template <int LPB>
+__device__ __forceinline__ void line_xy(int x0, int y0, int l, const Shape& sh, int& x, int& y) {
+  if (LPB <= sh.ny) line_xy_<LPB, true>(x0, y0, l, sh, x, y);
+  else line_xy_<LPB, false>(x0, y0, l, sh, x, y);
 }
 
 // n / D for 0 <= n < NMAX as one full-rate 24-bit multiply and a shift (the compiler's
@@ -976,10 +983,10 @@ __device__ __forceinline__ void z_stage_put(const ZStage<M, ZV>& st, const Shape
 // issue the loads of the block's spectra into registers: bins k < M of each line from its
 // row (thread-to-element mapping of the real volumes), and the Nyquist bin k = M of line
 // line0 + threadIdx.x from the side plane
-template <int M, int ZV>
-__device__ __forceinline__ void z_spec_load(gptr<const float2> spec, long line0, int nl,
-                                            const Shape& sh, float2 (&r)[ZCount<M, ZV>::EPT],
-                                            float2& rn) {
+template <int M, int ZV, bool NARROW>
+__device__ __forceinline__ void z_spec_load_(gptr<const float2> spec, long line0, int nl,
+                                             const Shape& sh, float2 (&r)[ZCount<M, ZV>::EPT],
+                                             float2& rn) {
   using E = ZElem<M, ZV>;
   const int x0 = (int)line0 / sh.ny, y0 = (int)line0 - x0 * sh.ny;  // block-uniform
   // row of line (x, y) = x ps + y ms = line ms + 16 x (ps = ny ms + 16): offsets from the
@@ -990,12 +997,22 @@ __device__ __forceinline__ void z_spec_load(gptr<const float2> spec, long line0,
     const int l0 = E::line(j), k = E::pos(j, l0);
     const int l = l0 < nl ? l0 : nl - 1;  // unconditional load (see z_stage_load)
     int x, y;
-    line_xy<ZPlan<M, ZV>::LPB>(x0, y0, l, sh, x, y);
+    line_xy_<ZPlan<M, ZV>::LPB, NARROW>(x0, y0, l, sh, x, y);
     // 24-bit multiply and unsigned sums: the offset stays a full-rate 32-bit computation
     const gptr<const float2> row = base + (__umul24((unsigned)l, (unsigned)sh.ms) + 16u * (unsigned)(x - x0));
     r[j] = row[k];
   }
   rn = spec[sh.so + line0 + ((int)threadIdx.x < nl ? (int)threadIdx.x : nl - 1)];
+}
+
+// the plane test of line_xy is block-uniform: taken once here, not per unrolled element
+// (per element it left every load in a basic block of its own behind a scalar branch)
+template <int M, int ZV>
+__device__ __forceinline__ void z_spec_load(gptr<const float2> spec, long line0, int nl,
+                                            const Shape& sh, float2 (&r)[ZCount<M, ZV>::EPT],
+                                            float2& rn) {
+  if (ZPlan<M, ZV>::LPB <= sh.ny) z_spec_load_<M, ZV, true>(spec, line0, nl, sh, r, rn);
+  else z_spec_load_<M, ZV, false>(spec, line0, nl, sh, r, rn);
 }
 
 template <int M, int ZV>
@@ -1267,12 +1284,14 @@ __global__ void __launch_bounds__((ZPlan<M, ZV>::THREADS))
   if (update) {
     // loads in the order they are consumed: staged tables, spectrum, source list (all
     // before the C2R), then the volumes (after it: their latency overlaps the transform)
+    // the block's source bounds first: they are block-uniform (scalar) loads, and the
+    // source entries' vector loads depend on them
+    const gptr<const int> sptr = G(m.src_ptr);
+    const int sbeg = sptr[line0], send = sptr[line0 + nl];
     ZStage<M, ZV> stg;
     z_stage_load<M, ZV>(twg, rtg, m, a.s, line0, nl, stg);
     float2 sr[CN::EPT], rc[CN::EPT], ab[CN::EPT], bt[NL ? CN::EPT : 1], srn;
     z_spec_load<M, ZV>(G(m.sp), line0, nl, a.s, sr, srn);
-    const gptr<const int> sptr = G(m.src_ptr);
-    const int sbeg = sptr[line0], send = sptr[line0 + nl];
     const bool src_fit = send - sbeg <= kSrcCap;  // block-uniform
     int sz_pf[kSrcPT];
     float2 sw_pf[kSrcPT];
