@@ -7,8 +7,9 @@ fuses into FMAs: their results differ at the float32 rounding level (2-3e-7 of t
 per step). Checked on nz = 256 and 384 (the z lengths of the benchmark workloads), 128
 (the serving shapes' 8 x 8 split, whose two-threads-per-row layout shares each stage-2
 DFT between two threads), 320 (a 16 x 10 split; like 384, one-wave blocks whose line
-buffers share LDS in layout 2) and 96 (an 8 x 6 split), with x/y lengths below 64 (one
-tile width) and at 64 and 128 (three),
+buffers share LDS in layout 2), 1024 (a 32 x 16 split whose layout-2 one-wave blocks both
+share LDS and pair their stage-2 DFTs) and 96 (an 8 x 6 split), with x/y lengths below 64
+(one tile width) and at 64 and 128 (three),
 heterogeneous, absorbing and nonlinear: one step from a random state under each pass's
 variants (the other passes at 0) against the float64 oracle (the one-step bar of
 tests/test_gpu_kspace.py); twenty more steps against the all-default combination (within
@@ -31,7 +32,7 @@ def _rel(a, b) -> float:
 
 
 @pytest.mark.parametrize("shape", [(16, 24, 256), (8, 16, 384), (8, 8, 320), (64, 128, 96),
-                                   (128, 64, 256), (16, 24, 128)])
+                                   (128, 64, 256), (16, 24, 128), (8, 8, 1024)])
 def test_variants_agree_and_match_oracle(shape):
     from caustica_amd.solvers.kspace.native import KSpaceSession
     from caustica_amd.solvers.kspace.operators import plan_member
