@@ -201,6 +201,20 @@ def serving_request(rid: int, rng: np.random.Generator, arrival_s: float, shape=
                         label=f"{solver} {'x'.join(map(str, shape))}")
 
 
+def serving_workload(seed: int, shape, solver: str) -> Workload:
+    """One request of the mixed-size queue as a Workload: the first request that
+    ``serving_request`` draws from numpy's generator seeded with ``seed``, at the given
+    shape and solver (its medium seed, steering offset and pinned schedule follow from the
+    seed). Used for the float64 parity fixtures of the queue's configurations."""
+    req = serving_request(0, np.random.default_rng(seed), 0.0, shape=tuple(shape), solver=solver)
+    tof = cw_tof_periods(req.grid.shape, req.grid.dx, req.medium.c_min, req.source.indices,
+                         req.source.f0, req.reference_point)
+    total = tof + req.spec.min_settle_periods + req.spec.n_record_periods
+    return Workload(f"serving_{solver}_{'x'.join(map(str, shape))}_seed{seed}", solver, req.grid,
+                    req.medium, req.source, req.spec, tuple(req.harmonics), total,
+                    reference_point=tuple(int(v) for v in req.reference_point))
+
+
 def serving_queue(n: int, rate_hz: float, seed: int = 4, shapes=None, progress=None):
     """n requests with Poisson arrivals at rate_hz (seeded exponential inter-arrival times);
     ``progress(done, n)`` (optional) is called after each request is built."""

@@ -4,10 +4,15 @@ Run in the build container (CPU only, no upstream import), as
 
     python tests/golden/make_fullsize.py cfg1 [--workers 4]
     python tests/golden/make_fullsize.py cfg2 [--workers 4]
+    python tests/golden/make_fullsize.py serving_w128x128x256 [--workers 4]
+    python tests/golden/make_fullsize.py serving_l192 [--workers 4]
 
 cfg1: the 256^3 heterogeneous linear solve (seeded layered + 2 % random-field medium,
       900 steps at spp 15), fundamental phasor;
 cfg2: the 256 x 256 x 384 Westervelt solve (2720 steps at spp 34), harmonics 1-3.
+serving_*: two requests of the mixed-size serving queue (steered bowl, fresh random-field
+      seed, pinned schedule): Westervelt 128 x 128 x 256 with harmonics 1-2 (975 steps)
+      and linear 192^3 (780 steps).
 The cfg1 run also writes full_cfg3.npz: the Pennes + CEM43 stage on the 256^3 grid,
 heated by the float64 fundamental phasor (Q = alpha |P|^2 / (rho c) scaled to 150 W/cm^2
 focal intensity, perfused tissue, dt 0.05 s, 200 heating + 400 cooling steps), by the
@@ -37,17 +42,28 @@ from caustica_amd import workloads  # noqa: E402
 from tests import _full  # noqa: E402
 
 
+# two requests of the mixed-size serving queue (workloads.serving_workload): a steered
+# Westervelt 128 x 128 x 256 solve with harmonics 1-2 and a steered linear 192^3 solve
+SERVING_CASES = {
+    "serving_w128x128x256": (41, (128, 128, 256), "westervelt"),
+    "serving_l192": (42, (192, 192, 192), "linear"),
+}
+
+
 def build(case: str):
     if case == "cfg1":
         return workloads.cfg1_hetero_linear(), dict(seed=1)
     if case == "cfg2":
         return workloads.cfg2_westervelt(), dict(seed=2)
+    if case in SERVING_CASES:
+        seed, shape, solver = SERVING_CASES[case]
+        return workloads.serving_workload(seed, shape, solver), dict(seed=seed)
     raise SystemExit(f"unknown case {case!r}")
 
 
 def main() -> None:
     ap = argparse.ArgumentParser()
-    ap.add_argument("case", choices=("cfg1", "cfg2"))
+    ap.add_argument("case", choices=("cfg1", "cfg2", *SERVING_CASES))
     ap.add_argument("--workers", type=int, default=4)
     args = ap.parse_args()
     t0 = time.time()

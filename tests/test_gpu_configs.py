@@ -13,8 +13,11 @@
 * one step from a random state at 256^3 and 256 x 256 x 384 under every thread-layout
   variant of the z passes, against the float64 oracle's step.
 * the Pennes two-step kernel on 256^3 (64-plane x chunks) against the float64 oracle.
+* two requests of the mixed-size serving queue (steered bowl, fresh random-field seed,
+  pinned schedule): Westervelt 128 x 128 x 256 with harmonics 1-2 and linear 192^3,
+  against float64 fixtures of the same requests (tests/golden/make_fullsize.py).
 
-Tolerances (north_star): fundamental phasor and focal peak rel <= 1e-4, harmonics 2-3
+Tolerances (the path's accuracy targets): fundamental phasor and focal peak rel <= 1e-4, harmonics 2-3
 rel-L2 <= 1e-3, p_max rel-L2 <= 1e-4, temperature <= 1e-3 K, CEM43 <= 1e-3 relative,
 one step <= 2e-5 of max |field|; schedules (spp, step counts, time of flight) equal and
 period peaks within 1e-4.
@@ -125,6 +128,23 @@ def test_cfg1_256_vs_fp64_fixture(cfg1_solved):
     _check_schedule(res, fx)
     _compare_samples(res.phasors[1], wl.reference_point, "ph1", fx, 1e-4)
     _compare_samples(res.p_max, wl.reference_point, "pmax", fx, 1e-4)
+
+
+@pytest.mark.parametrize("case", ["serving_w128x128x256", "serving_l192"])
+def test_serving_request_vs_fp64_fixture(case):
+    from caustica_amd import workloads
+
+    fx = _fixture(case)
+    m = fx["meta"]
+    wl = workloads.serving_workload(m["seed"], tuple(m["shape"]), m["solver"])
+    _check_medium(wl, fx)
+    print(f"[configs] {wl.name}: hip solve", flush=True)
+    res = _solve_hip(wl)
+    _check_schedule(res, fx)
+    _compare_samples(res.phasors[1], wl.reference_point, "ph1", fx, 1e-4)
+    _compare_samples(res.p_max, wl.reference_point, "pmax", fx, 1e-4)
+    for h in wl.harmonics[1:]:
+        _compare_samples(res.phasors[h], wl.reference_point, f"ph{h}", fx, 1e-3)
 
 
 def test_cfg2_westervelt_vs_fp64_fixture():
