@@ -51,3 +51,19 @@ def test_fixture_inputs_rebuild(case):
     assert (spp, tof) == (meta["spp"], meta["tof_periods"])
     assert dt == pytest.approx(meta["dt"], rel=1e-15)
     assert wl.periods_total * spp == meta["steps_total"]
+
+
+def test_serving_workload_is_deterministic():
+    from caustica_amd import workloads
+
+    a = workloads.serving_workload(7, (64, 64, 96), "westervelt")
+    b = workloads.serving_workload(7, (64, 64, 96), "westervelt")
+    c = workloads.serving_workload(8, (64, 64, 96), "westervelt")
+    assert a.name == b.name and a.periods_total == b.periods_total and a.harmonics == (1, 2)
+    for k in ("c", "rho", "alpha", "beta"):
+        assert np.array_equal(getattr(a.medium, k), getattr(b.medium, k)), k
+    np.testing.assert_array_equal(a.source.indices, b.source.indices)
+    np.testing.assert_array_equal(a.source.phases, b.source.phases)
+    assert a.reference_point == b.reference_point
+    # another seed draws another random field and steering offset
+    assert not np.array_equal(a.medium.c, c.medium.c)
